@@ -1,0 +1,140 @@
+/* tissuemc_hip.h — C ABI of the MI355X photon-transport engine.
+ *
+ * Plain C types only (no C++ or torch types), so that any FFI can bind it (ctypes, cgo, JNI,
+ * N-API). The library owns all device memory and streams; the caller owns every host buffer it
+ * passes in, and no pointer is retained after a call returns. Calls are synchronous and
+ * blocking; they are safe to make one after another from one thread. Errors never cross the ABI
+ * as exceptions: every entry point returns a TMC_* status and tmc_last_error() gives the message
+ * (per calling thread).
+ *
+ * Reference interface replaced (Raghava-Dwivedi/tissuemc):
+ *   tmc_hip_available    <- bool cuda_available()                      include/tissuemc/cuda_engine.hpp:9
+ *   tmc_hip_run          <- RunResult run_cuda(const RunSpec&,
+ *                                               const EngineOptions&)  include/tissuemc/cuda_engine.hpp:14
+ *   tmc_spec             <- struct RunSpec / LayerSpec                 include/tissuemc/config.hpp:14-29
+ *   tmc_opts             <- struct EngineOptions                       include/tissuemc/cpu_engine.hpp:18-30
+ *   tmc_out              <- struct RunResult / BatchResult             include/tissuemc/result.hpp:15-46
+ *   tmc_parse_mci_file   <- parse_mci_file(const std::string&)         include/tissuemc/mci.hpp:21
+ *   tmc_validate_spec    <- validate(const RunSpec&)                   include/tissuemc/config.hpp:38
+ *   TMC_ERR_CONFIG       <- ConfigError                                include/tissuemc/config.hpp:31
+ *   TMC_ERR_DEVICE/RUNTIME <- std::runtime_error (no device, API error) src/cuda/engine.cu:18-26,351
+ *
+ * Tallies are integers in units of 2^-32 of one launched photon's weight (tmc_fixed_point_unit).
+ * Integer sums are exact and associative, so a given (input, seed, photons, batches) gives
+ * bit-identical tallies on every run, whatever the launch shape.
+ */
+#ifndef TISSUEMC_HIP_H
+#define TISSUEMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMC_MAX_LAYERS 16
+
+enum {
+    TMC_OK = 0,
+    TMC_ERR_CONFIG = 1,  /* invalid input or options (the reference's ConfigError) */
+    TMC_ERR_DEVICE = 2,  /* no usable GPU */
+    TMC_ERR_RUNTIME = 3  /* a HIP API call failed */
+};
+
+/* Option flags (tmc_opts.flags). */
+#define TMC_FLAG_ARZ 1u           /* also tally A(r, z) into tmc_out.arz */
+#define TMC_FLAG_NO_PROFILES 2u   /* diagnostic: skip the A_z/R_r/T_r/A_rz profiles (zeros) */
+
+typedef struct tmc_layer {
+    double n;   /* refractive index */
+    double mua; /* absorption coefficient [1/cm] */
+    double mus; /* scattering coefficient [1/cm] */
+    double g;   /* anisotropy */
+    double d;   /* thickness [cm] */
+} tmc_layer;
+
+/* One MCML run, as read from an .mci file. */
+typedef struct tmc_spec {
+    uint64_t photons;
+    double dz, dr;          /* grid bin widths [cm] */
+    int32_t nz, nr, na;     /* grid bin counts (na is accepted and not used) */
+    double n_above, n_below;
+    int32_t num_layers;
+    tmc_layer layers[TMC_MAX_LAYERS];
+    char output_name[128];
+} tmc_spec;
+
+typedef struct tmc_opts {
+    uint64_t seed;           /* 64-bit Philox key */
+    int32_t batches;         /* >= 1 and <= photons */
+    int32_t device;          /* HIP device ordinal */
+    uint64_t photons;        /* nonzero overrides spec->photons */
+    uint32_t stream;         /* Philox counter word 3; 0 gives the reference's streams */
+    uint32_t flags;          /* TMC_FLAG_* */
+    int32_t waves_per_group; /* 0 = automatic */
+    int32_t groups;          /* persistent workgroups; 0 = one per compute unit */
+    int32_t chunk;           /* photon ids claimed per wave at a time; 0 = automatic */
+    int32_t arz_tile_z;      /* depth bins of A(r,z) kept in LDS; -1 = automatic */
+    int32_t arz_tile_r;      /* radial bins of A(r,z) kept in LDS; -1 = automatic */
+    uint64_t step_limit;     /* steps after which a photon is ended as an anomaly; 0 = 1e9 */
+} tmc_opts;
+
+/* Results. Array pointers are caller-owned host buffers sized by B = batches, L = num_layers,
+ * nz and nr; all values are in units of tmc_fixed_point_unit() except the counters. */
+typedef struct tmc_out {
+    uint64_t *rd;             /* [B]    diffuse reflectance weight per batch */
+    uint64_t *tt;             /* [B]    transmitted weight per batch */
+    uint64_t *absorbed;       /* [B*L]  absorbed weight per batch and layer */
+    uint64_t *events;         /* [B]    absorption events */
+    uint64_t *anomalies;      /* [B]    non-finite weights, non-unit directions, step limit */
+    uint64_t *boundary_steps; /* [B]    steps that ended on a layer boundary */
+    uint64_t *az;             /* [B*nz] absorbed weight per batch and depth bin */
+    uint64_t *rr;             /* [B*nr] diffuse reflectance per batch and radial bin */
+    uint64_t *tr;             /* [B*nr] transmittance per batch and radial bin */
+    uint64_t *arz;            /* [nz*nr] absorbed weight per (z, r) bin, all batches; or NULL */
+    double specular;          /* specular reflectance of the launched beam */
+    double kernel_seconds;    /* transport kernel time (HIP events) */
+    double wall_seconds;      /* whole call */
+    char device[96];          /* device name and architecture */
+    int32_t groups, waves_per_group, lds_bytes, chunk, arz_tile_z, arz_tile_r;
+} tmc_out;
+
+/* 1 if a HIP device is present and usable, else 0. */
+int tmc_hip_available(void);
+
+/* Simulate one run on the GPU (blocking). Batch b covers photon ids
+ * [b*base + min(b, extra), ...) exactly as the reference's BatchPlan, and photon i uses the
+ * Philox stream keyed by (seed, i), so per-batch results are comparable photon for photon with
+ * the reference's CPU engine. */
+int tmc_hip_run(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *tmc_last_error(void);
+
+/* 2^-32: the weight of one tally unit. */
+double tmc_fixed_point_unit(void);
+
+/* Fill opts with the defaults (seed 1, 10 batches, device 0, automatic launch shape). */
+void tmc_default_opts(tmc_opts *opts);
+
+/* Check a run description like the reference's validate(); TMC_ERR_CONFIG names the field. */
+int tmc_validate_spec(const tmc_spec *spec);
+
+/* Parse an MCML .mci file into up to max_runs runs; *n_runs receives the count. Errors name the
+ * file and line (TMC_ERR_CONFIG). */
+int tmc_parse_mci_file(const char *path, tmc_spec *runs, int32_t max_runs, int32_t *n_runs);
+
+/* The same from the text of an .mci file (<- parse_mci(std::istream&), include/tissuemc/mci.hpp:18). */
+int tmc_parse_mci_text(const char *text, tmc_spec *runs, int32_t max_runs, int32_t *n_runs);
+
+/* The first n 32-bit words of the photon stream (seed, photon_id, word3), generated by the
+ * device's Philox4x32-10 code path. A parity hook: the stream must equal the reference's
+ * PhotonRng word for word (include/tissuemc/philox.hpp:62-83). */
+int tmc_hip_philox_stream(uint64_t seed, uint64_t photon_id, uint32_t word3, int64_t n,
+                          uint32_t *out, int32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TISSUEMC_HIP_H */

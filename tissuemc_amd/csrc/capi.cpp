@@ -1,0 +1,269 @@
+// The C ABI (include/tissuemc_hip.h): error mapping, run validation and the .mci reader.
+//
+// The .mci reader accepts the MCML input format the reference reads (src/io/mci.cpp:97-171):
+// '#' starts a comment, blank lines are skipped, values come in a fixed order, counts must be
+// integers, and every error names its line. Validation rules and messages follow
+// include/tissuemc/config.hpp:38-66.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F> int guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return TMC_OK;
+    } catch (const tmc::EngineError &e) {
+        return set_error(e.code(), e.what());
+    } catch (const std::bad_alloc &) {
+        return set_error(TMC_ERR_RUNTIME, "out of host memory");
+    } catch (const std::exception &e) {
+        return set_error(TMC_ERR_RUNTIME, e.what());
+    }
+}
+
+[[noreturn]] void config_fail(const std::string &msg) { throw tmc::EngineError(TMC_ERR_CONFIG, msg); }
+
+// ------------------------------------------------------------------ .mci reader
+
+struct McLine {
+    int number;
+    std::vector<std::string> words;
+};
+
+[[noreturn]] void line_fail(int line, const std::string &msg) {
+    config_fail("line " + std::to_string(line) + ": " + msg);
+}
+
+class McReader {
+  public:
+    explicit McReader(const std::string &text) {
+        std::istringstream in(text);
+        std::string raw;
+        int number = 0;
+        while (std::getline(in, raw)) {
+            ++number;
+            const size_t hash = raw.find('#');
+            if (hash != std::string::npos) raw.resize(hash);
+            std::istringstream ws(raw);
+            McLine l{number, {}};
+            for (std::string w; ws >> w;) l.words.push_back(w);
+            if (!l.words.empty()) lines_.push_back(std::move(l));
+        }
+    }
+
+    const McLine &take(size_t count, const char *what) {
+        if (pos_ >= lines_.size()) {
+            const int last = lines_.empty() ? 0 : lines_.back().number;
+            line_fail(last, std::string("unexpected end of file, expected ") + what);
+        }
+        const McLine &l = lines_[pos_++];
+        if (l.words.size() != count)
+            line_fail(l.number, std::string(what) + ": expected " + std::to_string(count) +
+                                    " value(s), found " + std::to_string(l.words.size()));
+        return l;
+    }
+
+    bool done() const { return pos_ >= lines_.size(); }
+    int next_line() const { return lines_[pos_].number; }
+
+  private:
+    std::vector<McLine> lines_;
+    size_t pos_ = 0;
+};
+
+double real_at(const McLine &l, size_t i, const std::string &what) {
+    const char *s = l.words[i].c_str();
+    char *end = nullptr;
+    errno = 0;
+    const double v = std::strtod(s, &end);
+    if (end == s || *end != '\0' || errno == ERANGE || !std::isfinite(v))
+        line_fail(l.number, what + ": '" + l.words[i] + "' is not a finite number");
+    return v;
+}
+
+long long int_at(const McLine &l, size_t i, const std::string &what) {
+    const char *s = l.words[i].c_str();
+    char *end = nullptr;
+    errno = 0;
+    const long long v = std::strtoll(s, &end, 10);
+    if (end == s || *end != '\0' || errno == ERANGE)
+        line_fail(l.number, what + ": '" + l.words[i] + "' is not an integer");
+    return v;
+}
+
+std::vector<tmc_spec> parse_text(const std::string &text) {
+    McReader rd(text);
+    const McLine &ver = rd.take(1, "file version");
+    if (real_at(ver, 0, "file version") != 1.0)
+        line_fail(ver.number, "unsupported file version '" + ver.words[0] + "' (expected 1.0)");
+    const McLine &nrl = rd.take(1, "number of runs");
+    const long long nruns = int_at(nrl, 0, "number of runs");
+    if (nruns < 1) line_fail(nrl.number, "number of runs must be >= 1");
+    std::vector<tmc_spec> runs;
+    for (long long r = 0; r < nruns; ++r) {
+        tmc_spec s;
+        std::memset(&s, 0, sizeof s);
+        const McLine &o = rd.take(2, "output filename and format");
+        std::snprintf(s.output_name, sizeof s.output_name, "%s", o.words[0].c_str());
+        if (o.words[1] != "A")
+            line_fail(o.number, "output format must be 'A' (ASCII), found '" + o.words[1] + "'");
+        const McLine &np = rd.take(1, "number of photons");
+        const long long photons = int_at(np, 0, "number of photons");
+        if (photons < 1) line_fail(np.number, "number of photons must be >= 1");
+        s.photons = static_cast<uint64_t>(photons);
+        const McLine &st = rd.take(2, "dz and dr");
+        s.dz = real_at(st, 0, "dz");
+        s.dr = real_at(st, 1, "dr");
+        const McLine &gr = rd.take(3, "grid counts (nz nr na)");
+        const long long nz = int_at(gr, 0, "nz"), nr = int_at(gr, 1, "nr"), na = int_at(gr, 2, "na");
+        constexpr long long kMaxBins = 1000000;
+        if (nz < 1 || nr < 1 || na < 1 || nz > kMaxBins || nr > kMaxBins || na > kMaxBins)
+            line_fail(gr.number, "grid counts must be integers in [1, 1000000]");
+        s.nz = static_cast<int32_t>(nz);
+        s.nr = static_cast<int32_t>(nr);
+        s.na = static_cast<int32_t>(na);
+        const McLine &nl = rd.take(1, "number of layers");
+        const long long layers = int_at(nl, 0, "number of layers");
+        if (layers < 1 || layers > tmc::kMaxLayers)
+            line_fail(nl.number, "number of layers must be in [1, " + std::to_string(tmc::kMaxLayers) + "]");
+        s.num_layers = static_cast<int32_t>(layers);
+        s.n_above = real_at(rd.take(1, "refractive index above"), 0, "n above");
+        for (long long i = 0; i < layers; ++i) {
+            const McLine &ll = rd.take(5, "layer (n mua mus g d)");
+            const std::string tag = "layer " + std::to_string(i + 1);
+            tmc_layer &L = s.layers[i];
+            L.n = real_at(ll, 0, tag + " n");
+            L.mua = real_at(ll, 1, tag + " mua");
+            L.mus = real_at(ll, 2, tag + " mus");
+            L.g = real_at(ll, 3, tag + " g");
+            L.d = real_at(ll, 4, tag + " thickness");
+        }
+        const McLine &below = rd.take(1, "refractive index below");
+        s.n_below = real_at(below, 0, "n below");
+        try {
+            tmc::validate_spec(s);
+        } catch (const tmc::EngineError &e) {
+            line_fail(below.number, std::string("invalid run: ") + e.what());
+        }
+        runs.push_back(s);
+    }
+    if (!rd.done()) line_fail(rd.next_line(), "unexpected extra content after the last run");
+    return runs;
+}
+
+void copy_runs(const std::vector<tmc_spec> &runs, tmc_spec *out, int32_t max_runs, int32_t *n_runs) {
+    if (n_runs) *n_runs = static_cast<int32_t>(runs.size());
+    if (static_cast<int64_t>(runs.size()) > max_runs)
+        config_fail("the file holds " + std::to_string(runs.size()) + " runs; room for " +
+                    std::to_string(max_runs));
+    for (size_t i = 0; i < runs.size(); ++i) out[i] = runs[i];
+}
+
+}  // namespace
+
+namespace tmc {
+
+void validate_spec(const tmc_spec &run) {
+    if (run.photons == 0) config_fail("number of photons must be > 0");
+    if (!(run.dz > 0) || !(run.dr > 0)) config_fail("dz and dr must be > 0");
+    if (run.nz < 1 || run.nr < 1 || run.na < 1) config_fail("grid bin counts must be >= 1");
+    if (!(run.n_above >= 1.0) || !(run.n_below >= 1.0))
+        config_fail("ambient refractive indices must be >= 1");
+    if (run.num_layers < 1) config_fail("at least one layer is required");
+    if (run.num_layers > kMaxLayers)
+        config_fail("at most " + std::to_string(kMaxLayers) + " layers are supported");
+    for (int i = 0; i < run.num_layers; ++i) {
+        const tmc_layer &L = run.layers[i];
+        const std::string where = "layer " + std::to_string(i + 1) + ": ";
+        if (!(L.n >= 1.0)) config_fail(where + "n must be >= 1");
+        if (!(L.mua >= 0) || !(L.mus >= 0)) config_fail(where + "mua and mus must be >= 0");
+        if (!(L.mua + L.mus > 0)) config_fail(where + "clear layers (mua = mus = 0) are not supported");
+        if (!(L.g > -1.0 && L.g < 1.0)) config_fail(where + "g must be in (-1, 1)");
+        if (!(L.d > 0) || !std::isfinite(L.d)) config_fail(where + "thickness must be finite and > 0");
+    }
+}
+
+}  // namespace tmc
+
+extern "C" {
+
+int tmc_hip_available(void) { return tmc::device_available() ? 1 : 0; }
+
+const char *tmc_last_error(void) { return g_last_error.c_str(); }
+
+double tmc_fixed_point_unit(void) { return 0x1p-32; }
+
+void tmc_default_opts(tmc_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->seed = 1;
+    o->batches = 10;
+    o->arz_tile_z = -1;
+    o->arz_tile_r = -1;
+}
+
+int tmc_validate_spec(const tmc_spec *spec) {
+    return guarded([&] {
+        if (!spec) config_fail("spec is NULL");
+        tmc::validate_spec(*spec);
+    });
+}
+
+int tmc_hip_run(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out) {
+    return guarded([&] {
+        if (!spec || !opts || !out) config_fail("spec, opts and out must not be NULL");
+        if (!out->rd || !out->tt || !out->absorbed || !out->events || !out->anomalies)
+            config_fail("tmc_out: rd, tt, absorbed, events and anomalies are required");
+        tmc::run_transport(*spec, *opts, out);
+    });
+}
+
+int tmc_parse_mci_text(const char *text, tmc_spec *runs, int32_t max_runs, int32_t *n_runs) {
+    return guarded([&] {
+        if (!text || !runs) config_fail("text and runs must not be NULL");
+        copy_runs(parse_text(text), runs, max_runs, n_runs);
+    });
+}
+
+int tmc_parse_mci_file(const char *path, tmc_spec *runs, int32_t max_runs, int32_t *n_runs) {
+    return guarded([&] {
+        if (!path || !runs) config_fail("path and runs must not be NULL");
+        std::ifstream f(path);
+        if (!f) config_fail(std::string("cannot open '") + path + "'");
+        std::stringstream ss;
+        ss << f.rdbuf();
+        try {
+            copy_runs(parse_text(ss.str()), runs, max_runs, n_runs);
+        } catch (const tmc::EngineError &e) {
+            throw tmc::EngineError(e.code(), std::string(path) + ": " + e.what());
+        }
+    });
+}
+
+int tmc_hip_philox_stream(uint64_t seed, uint64_t photon_id, uint32_t word3, int64_t n,
+                          uint32_t *out, int32_t device) {
+    return guarded([&] {
+        if (!out && n > 0) config_fail("out must not be NULL");
+        tmc::philox_stream(seed, photon_id, word3, n, out, device);
+    });
+}
+
+}  // extern "C"

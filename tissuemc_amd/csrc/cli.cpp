@@ -1,0 +1,180 @@
+// `tissuemc` command-line host: the reference's `run` surface (src/cli/main.cpp:18-204) on the
+// MI355X engine. Same input (.mci), same summary lines, same JSON (a superset), same exit
+// codes: 0 ok, 1 error or anomalies, 2 usage.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <fstream>
+#include <string>
+
+#include "run_result.h"
+
+namespace {
+
+void usage(std::FILE *to) {
+    std::fprintf(to,
+                 "usage:\n"
+                 "  tissuemc --version\n"
+                 "  tissuemc run <input.mci> [options]\n"
+                 "\n"
+                 "options for 'run':\n"
+                 "  --photons N          override the photon count in the input file\n"
+                 "  --batches B          number of batches for the standard error (default 10)\n"
+                 "  --seed S             64-bit random seed (default 1)\n"
+                 "  --engine E           hip (default): the MI355X engine\n"
+                 "  --precision P        float32 (the GPU transport precision)\n"
+                 "  --device D           HIP device ordinal (default 0)\n"
+                 "  --arz                also tally A(r,z)\n"
+                 "  --no-profiles        diagnostic: skip the depth and radius profiles\n"
+                 "  --waves-per-group W  launch shape (default automatic)\n"
+                 "  --groups G           persistent workgroups (default one per compute unit)\n"
+                 "  --chunk C            photon ids claimed per wave at a time\n"
+                 "  --arz-tile Z R       A(r,z) bins kept in LDS (default automatic)\n"
+                 "  --stream W           Philox counter word 3 (default 0, the reference streams)\n"
+                 "  --output FILE        write full results as JSON\n");
+}
+
+unsigned long long parse_count(const char *name, const char *text) {
+    char *end = nullptr;
+    const unsigned long long v = std::strtoull(text, &end, 10);
+    if (end == text || *end != '\0' || text[0] == '-') {
+        std::fprintf(stderr, "tissuemc: %s expects a non-negative integer, got '%s'\n", name, text);
+        std::exit(2);
+    }
+    return v;
+}
+
+int run_command(int argc, char **argv) {
+    if (argc < 3) {
+        usage(stderr);
+        return 2;
+    }
+    const std::string input = argv[2];
+    tmc_opts opt;
+    tmc_default_opts(&opt);
+    std::string output;
+    for (int i = 3; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto value = [&](const char *name) -> const char * {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "tissuemc: %s needs a value\n", name);
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--photons") {
+            opt.photons = parse_count("--photons", value("--photons"));
+        } else if (a == "--batches") {
+            opt.batches = static_cast<int32_t>(parse_count("--batches", value("--batches")));
+        } else if (a == "--seed") {
+            opt.seed = parse_count("--seed", value("--seed"));
+        } else if (a == "--engine") {
+            const std::string e = value("--engine");
+            if (e != "hip") {
+                std::fprintf(stderr,
+                             "tissuemc: --engine must be hip, got '%s' (this build is the MI355X "
+                             "engine; the float64 CPU engine is the test oracle)\n",
+                             e.c_str());
+                return 2;
+            }
+        } else if (a == "--precision") {
+            const std::string p = value("--precision");
+            if (p != "float32") {
+                std::fprintf(stderr, "tissuemc: the GPU transports in float32, got '%s'\n", p.c_str());
+                return 2;
+            }
+        } else if (a == "--device") {
+            opt.device = static_cast<int32_t>(parse_count("--device", value("--device")));
+        } else if (a == "--arz") {
+            opt.flags |= TMC_FLAG_ARZ;
+        } else if (a == "--no-profiles") {
+            opt.flags |= TMC_FLAG_NO_PROFILES;
+        } else if (a == "--waves-per-group") {
+            opt.waves_per_group = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--groups") {
+            opt.groups = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--chunk") {
+            opt.chunk = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--arz-tile") {
+            opt.arz_tile_z = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
+            opt.arz_tile_r = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--stream") {
+            opt.stream = static_cast<uint32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--output") {
+            output = value("--output");
+        } else {
+            std::fprintf(stderr, "tissuemc: unknown option '%s'\n", a.c_str());
+            usage(stderr);
+            return 2;
+        }
+    }
+    if ((opt.flags & TMC_FLAG_ARZ) && (opt.flags & TMC_FLAG_NO_PROFILES)) {
+        std::fprintf(stderr, "tissuemc: --arz and --no-profiles cannot be combined\n");
+        return 2;
+    }
+
+    const auto runs = tmc::parse_mci_file(input);
+    std::ofstream json;
+    if (!output.empty()) {
+        json.open(output);
+        if (!json) {
+            std::fprintf(stderr, "tissuemc: cannot write '%s'\n", output.c_str());
+            return 1;
+        }
+        json << "{\"runs\":[";
+    }
+    int status = 0;
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const tmc::RunResult res = tmc::run_hip(runs[r], opt);
+        const auto rd = res.rd(), tt = res.tt(), ab = res.absorbed();
+        const auto cons = res.conservation_residual();
+        std::printf("run %zu: %llu photons, %zu batches, %.2f s (%s, %s)\n", r + 1,
+                    static_cast<unsigned long long>(res.photons()), res.batches.size(),
+                    res.wall_seconds, res.engine.c_str(), res.precision.c_str());
+        std::printf("  device %s, kernel %.3f s, %s totals\n", res.device.c_str(), res.kernel_seconds,
+                    res.tally_precision.c_str());
+        std::printf("  specular R_sp = %.6f\n", res.specular);
+        std::printf("  diffuse  R_d  = %.6f +/- %.6f\n", rd.mean, rd.se);
+        std::printf("  transmit T_t  = %.6f +/- %.6f\n", tt.mean, tt.se);
+        std::printf("  absorbed  A   = %.6f +/- %.6f\n", ab.mean, ab.se);
+        std::printf("  conservation residual = %+.2e +/- %.2e\n", cons.mean, cons.se);
+        if (res.anomalies() != 0) {
+            std::fprintf(stderr,
+                         "tissuemc: %llu anomalies (non-finite weight, non-unit direction or step "
+                         "limit); results are not trustworthy\n",
+                         static_cast<unsigned long long>(res.anomalies()));
+            status = 1;
+        }
+        if (json.is_open()) {
+            if (r) json << ',';
+            tmc::write_json(res, json);
+        }
+    }
+    if (json.is_open()) {
+        json << "]}\n";
+        json.flush();
+        if (!json) {
+            std::fprintf(stderr, "tissuemc: error writing '%s'\n", output.c_str());
+            return 1;
+        }
+    }
+    return status;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    try {
+        if (argc >= 2 && std::strcmp(argv[1], "--version") == 0) {
+            std::printf("tissuemc 0.1.0 (MI355X engine)\n");
+            return 0;
+        }
+        if (argc >= 2 && std::strcmp(argv[1], "run") == 0) return run_command(argc, argv);
+        usage(stderr);
+        return 2;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "tissuemc: error: %s\n", e.what());
+        return 1;
+    }
+}

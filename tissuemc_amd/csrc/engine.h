@@ -1,0 +1,35 @@
+// Internal C++ interface between the C ABI (capi.cpp) and the HIP engine (transport.hip).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "tissuemc_hip.h"
+
+namespace tmc {
+
+constexpr int kMaxLayers = TMC_MAX_LAYERS;
+
+// An error with the C-ABI status it maps to (TMC_ERR_CONFIG / _DEVICE / _RUNTIME).
+class EngineError : public std::runtime_error {
+  public:
+    EngineError(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+  private:
+    int code_;
+};
+
+// Throws EngineError(TMC_ERR_CONFIG) naming the offending field, as the reference's
+// validate() does (include/tissuemc/config.hpp:38-66).
+void validate_spec(const tmc_spec &spec);
+
+// Runs the transport kernel for one run and fills `out` (capi.cpp checks pointers first).
+void run_transport(const tmc_spec &spec, const tmc_opts &opts, tmc_out *out);
+
+// Device-side Philox stream, for the parity hook.
+void philox_stream(uint64_t seed, uint64_t id, uint32_t word3, int64_t n, uint32_t *out, int device);
+
+bool device_available();
+
+}  // namespace tmc

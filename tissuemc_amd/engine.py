@@ -1,0 +1,278 @@
+"""Python binding of the MI355X engine's C ABI (include/tissuemc_hip.h) through ctypes.
+
+``run()`` returns a dict in the reference's result schema (src/io/json_out.cpp:164-306) with the
+GPU's extras. There is no fallback: if the HIP library is missing or no GPU is present, the call
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libtissuemc_hip.so"
+MAX_LAYERS = 16
+UNIT = 2.0 ** -32  # weight of one tally unit
+
+TMC_OK, TMC_ERR_CONFIG, TMC_ERR_DEVICE, TMC_ERR_RUNTIME = 0, 1, 2, 3
+FLAG_ARZ, FLAG_NO_PROFILES = 1, 2
+
+
+class ConfigError(ValueError):
+    """Invalid input or options (the reference's ConfigError)."""
+
+
+class DeviceError(RuntimeError):
+    """No usable GPU, or a HIP call failed."""
+
+
+class TmcLayer(C.Structure):
+    _fields_ = [("n", C.c_double), ("mua", C.c_double), ("mus", C.c_double), ("g", C.c_double),
+                ("d", C.c_double)]
+
+
+class TmcSpec(C.Structure):
+    _fields_ = [("photons", C.c_uint64), ("dz", C.c_double), ("dr", C.c_double),
+                ("nz", C.c_int32), ("nr", C.c_int32), ("na", C.c_int32),
+                ("n_above", C.c_double), ("n_below", C.c_double), ("num_layers", C.c_int32),
+                ("layers", TmcLayer * MAX_LAYERS), ("output_name", C.c_char * 128)]
+
+
+class TmcOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("batches", C.c_int32), ("device", C.c_int32),
+                ("photons", C.c_uint64), ("stream", C.c_uint32), ("flags", C.c_uint32),
+                ("waves_per_group", C.c_int32), ("groups", C.c_int32), ("chunk", C.c_int32),
+                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("step_limit", C.c_uint64)]
+
+
+_PU64 = C.POINTER(C.c_uint64)
+
+
+class TmcOut(C.Structure):
+    _fields_ = [("rd", _PU64), ("tt", _PU64), ("absorbed", _PU64), ("events", _PU64),
+                ("anomalies", _PU64), ("boundary_steps", _PU64), ("az", _PU64), ("rr", _PU64),
+                ("tr", _PU64), ("arz", _PU64), ("specular", C.c_double),
+                ("kernel_seconds", C.c_double), ("wall_seconds", C.c_double),
+                ("device", C.c_char * 96), ("groups", C.c_int32), ("waves_per_group", C.c_int32),
+                ("lds_bytes", C.c_int32), ("chunk", C.c_int32), ("arz_tile_z", C.c_int32),
+                ("arz_tile_r", C.c_int32)]
+
+
+EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_point_unit",
+            "tmc_default_opts", "tmc_validate_spec", "tmc_parse_mci_file", "tmc_parse_mci_text",
+            "tmc_hip_philox_stream")
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """The engine library. Raises if it has not been built: there is no fallback path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise DeviceError(f"the HIP engine library is not built ({LIB_PATH}); run "
+                          "__graft_entry__.build() or `make -C tissuemc_amd`")
+    L = C.CDLL(str(LIB_PATH))
+    L.tmc_hip_available.restype = C.c_int
+    L.tmc_hip_run.argtypes = [C.POINTER(TmcSpec), C.POINTER(TmcOpts), C.POINTER(TmcOut)]
+    L.tmc_hip_run.restype = C.c_int
+    L.tmc_last_error.restype = C.c_char_p
+    L.tmc_fixed_point_unit.restype = C.c_double
+    L.tmc_default_opts.argtypes = [C.POINTER(TmcOpts)]
+    L.tmc_validate_spec.argtypes = [C.POINTER(TmcSpec)]
+    L.tmc_validate_spec.restype = C.c_int
+    L.tmc_parse_mci_file.argtypes = [C.c_char_p, C.POINTER(TmcSpec), C.c_int32,
+                                     C.POINTER(C.c_int32)]
+    L.tmc_parse_mci_file.restype = C.c_int
+    L.tmc_parse_mci_text.argtypes = [C.c_char_p, C.POINTER(TmcSpec), C.c_int32,
+                                     C.POINTER(C.c_int32)]
+    L.tmc_parse_mci_text.restype = C.c_int
+    L.tmc_hip_philox_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64,
+                                        C.POINTER(C.c_uint32), C.c_int32]
+    L.tmc_hip_philox_stream.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _raise(code: int) -> None:
+    msg = load().tmc_last_error().decode()
+    if code == TMC_ERR_CONFIG:
+        raise ConfigError(msg)
+    raise DeviceError(msg)
+
+
+def available() -> bool:
+    return bool(load().tmc_hip_available())
+
+
+# ------------------------------------------------------------------ specs
+
+def spec_to_c(spec: dict) -> TmcSpec:
+    s = TmcSpec()
+    s.photons = int(spec["photons"])
+    s.dz, s.dr = float(spec["dz"]), float(spec["dr"])
+    s.nz, s.nr, s.na = int(spec["nz"]), int(spec["nr"]), int(spec.get("na", 1))
+    s.n_above, s.n_below = float(spec["n_above"]), float(spec["n_below"])
+    layers = spec["layers"]
+    if len(layers) > MAX_LAYERS:
+        raise ConfigError(f"at most {MAX_LAYERS} layers are supported")
+    s.num_layers = len(layers)
+    for i, L in enumerate(layers):
+        s.layers[i] = TmcLayer(L["n"], L["mua"], L["mus"], L["g"], L["d"])
+    s.output_name = str(spec.get("output_name", "")).encode()[:127]
+    return s
+
+
+def spec_from_c(s: TmcSpec) -> dict:
+    return {
+        "output_name": s.output_name.decode(), "photons": int(s.photons), "dz": s.dz, "dr": s.dr,
+        "nz": s.nz, "nr": s.nr, "na": s.na, "n_above": s.n_above, "n_below": s.n_below,
+        "layers": [{"n": s.layers[i].n, "mua": s.layers[i].mua, "mus": s.layers[i].mus,
+                    "g": s.layers[i].g, "d": s.layers[i].d} for i in range(s.num_layers)],
+    }
+
+
+def parse_mci(text: str | None = None, *, path: str | Path | None = None) -> list[dict]:
+    """Runs of an .mci file (by path or text), via the library's reader."""
+    L = load()
+    runs = (TmcSpec * 64)()
+    n = C.c_int32(0)
+    if path is not None:
+        code = L.tmc_parse_mci_file(str(path).encode(), runs, 64, C.byref(n))
+    else:
+        code = L.tmc_parse_mci_text(text.encode(), runs, 64, C.byref(n))
+    if code != TMC_OK:
+        _raise(code)
+    return [spec_from_c(runs[i]) for i in range(n.value)]
+
+
+def validate_spec(spec: dict) -> None:
+    code = load().tmc_validate_spec(C.byref(spec_to_c(spec)))
+    if code != TMC_OK:
+        _raise(code)
+
+
+# ------------------------------------------------------------------ runs
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_PU64)
+
+
+def batch_estimate(sums, photons) -> tuple[float, float]:
+    """(mean, se) of the per-photon value from per-batch sums (json_out.cpp:37-64)."""
+    sums = np.asarray(sums, dtype=np.float64)
+    photons = np.asarray(photons, dtype=np.float64)
+    mean = float(sums.sum() / photons.sum())
+    n = len(sums)
+    if n < 2:
+        return mean, float("nan")
+    per = sums / photons
+    return mean, float(np.sqrt(((per - per.mean()) ** 2).sum() / (n - 1) / n))
+
+
+def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
+        arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
+        groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
+        step_limit: int = 0) -> dict:
+    """Simulate one run on the GPU; returns the reference's result schema plus extras.
+
+    Extras: ``t_r``, ``a_rz`` (if ``arz``), ``profile_se`` (per-bin standard errors), per-batch
+    profiles ``az_b``/``rr_b``/``tr_b`` and the raw integer tallies under ``fixed``.
+    """
+    L = load()
+    cs = spec_to_c(spec)
+    n_ph = photons or int(spec["photons"])
+    B, NL, nz, nr = int(batches), cs.num_layers, cs.nz, cs.nr
+    o = TmcOpts()
+    L.tmc_default_opts(C.byref(o))
+    o.seed, o.batches, o.device, o.photons, o.stream = seed, B, device, n_ph, stream
+    o.flags = (FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
+    o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
+    if arz_tile is not None:
+        o.arz_tile_z, o.arz_tile_r = arz_tile
+    if B < 1:
+        raise ConfigError("batches must be >= 1")
+    rd, tt, ev, an, bs = (np.zeros(B, np.uint64) for _ in range(5))
+    ab = np.zeros(B * NL, np.uint64)
+    az = np.zeros(B * nz, np.uint64)
+    rr = np.zeros(B * nr, np.uint64)
+    tr = np.zeros(B * nr, np.uint64)
+    a_rz = np.zeros(nz * nr, np.uint64) if arz else None
+    out = TmcOut(rd=_p(rd), tt=_p(tt), absorbed=_p(ab), events=_p(ev), anomalies=_p(an),
+                 boundary_steps=_p(bs), az=_p(az), rr=_p(rr), tr=_p(tr),
+                 arz=_p(a_rz) if arz else None)
+    code = L.tmc_hip_run(C.byref(cs), C.byref(o), C.byref(out))
+    if code != TMC_OK:
+        _raise(code)
+
+    base, extra = divmod(n_ph, B)
+    firsts = [b * base + min(b, extra) for b in range(B)]
+    counts = [base + (1 if b < extra else 0) for b in range(B)]
+    az_b = az.reshape(B, nz)
+    rr_b = rr.reshape(B, nr)
+    tr_b = tr.reshape(B, nr)
+
+    def prof(pb: np.ndarray) -> tuple[list[float], list[float]]:
+        tot = [float(v) * UNIT for v in pb.sum(axis=0, dtype=np.uint64)]
+        per = pb.astype(np.float64) * UNIT / np.asarray(counts, np.float64)[:, None]
+        se = (per.std(axis=0, ddof=1) / np.sqrt(B)).tolist() if B > 1 else [float("nan")] * pb.shape[1]
+        return tot, se
+
+    a_z, a_z_se = prof(az_b)
+    rd_r, rd_r_se = prof(rr_b)
+    t_r, t_r_se = prof(tr_b)
+    batches_out = [
+        {"first_photon": firsts[b], "photons": counts[b], "rd": float(rd[b]) * UNIT,
+         "tt": float(tt[b]) * UNIT,
+         "absorbed": [float(x) * UNIT for x in ab[b * NL:(b + 1) * NL]],
+         "events": int(ev[b]), "anomalies": int(an[b]), "boundary_steps": int(bs[b])}
+        for b in range(B)
+    ]
+    res = {
+        "engine": "hip", "precision": "float32", "seed": seed, "threads": 1,
+        "photons": int(sum(counts)), "output_name": spec.get("output_name", ""),
+        "n_above": cs.n_above, "n_below": cs.n_below,
+        "layers": [dict(x) for x in spec["layers"]],
+        "grid": {"dz": cs.dz, "dr": cs.dr, "nz": nz, "nr": nr},
+        "specular": out.specular, "wall_seconds": out.wall_seconds,
+        "kernel_seconds": out.kernel_seconds, "device": out.device.decode(),
+        "tally_precision": "int64 fixed point (2^-32)", "rung": 0, "profiles": profiles,
+        "events": int(ev.sum()), "anomalies": int(an.sum()), "boundary_steps": int(bs.sum()),
+        "launch": {"groups": out.groups, "waves_per_group": out.waves_per_group,
+                   "lds_bytes": out.lds_bytes, "chunk": out.chunk,
+                   "arz_tile": [out.arz_tile_z, out.arz_tile_r]},
+        "batches": batches_out,
+        "a_z": a_z, "rd_r": rd_r, "t_r": t_r,
+        "profile_se": {"a_z": a_z_se, "rd_r": rd_r_se, "t_r": t_r_se},
+        "az_b": az_b, "rr_b": rr_b, "tr_b": tr_b,
+        "fixed": {"rd": rd, "tt": tt, "absorbed": ab.reshape(B, NL), "az": az_b, "rr": rr_b,
+                  "tr": tr_b, "arz": a_rz.reshape(nz, nr) if arz else None},
+    }
+    photons_b = [bb["photons"] for bb in batches_out]
+    summ = {}
+    for key in ("rd", "tt"):
+        m, s = batch_estimate([bb[key] for bb in batches_out], photons_b)
+        summ[key] = {"mean": m, "se": s}
+    m, s = batch_estimate([sum(bb["absorbed"]) for bb in batches_out], photons_b)
+    summ["absorbed"] = {"mean": m, "se": s}
+    m, s = batch_estimate([bb["rd"] + bb["tt"] + sum(bb["absorbed"]) for bb in batches_out],
+                          photons_b)
+    summ["conservation_residual"] = {"mean": m + out.specular - 1.0, "se": s}
+    res["summary"] = summ
+    if arz:
+        res["a_rz"] = (a_rz.astype(np.float64) * UNIT).reshape(nz, nr)
+    return res
+
+
+def philox_stream(seed: int, photon_id: int, n: int, word3: int = 0, device: int = 0) -> np.ndarray:
+    """The first n words of the device's Philox stream for (seed, photon_id, word3)."""
+    out = np.zeros(n, np.uint32)
+    code = load().tmc_hip_philox_stream(seed, photon_id, word3, n,
+                                        out.ctypes.data_as(C.POINTER(C.c_uint32)), device)
+    if code != TMC_OK:
+        _raise(code)
+    return out
