@@ -1,0 +1,175 @@
+"""Throughput of the MI355X photon-transport engine on the three-layer workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--no-arz]
+
+One step is one complete run of the workload (three-layer stack, 1e8 photons, 200 x 200 grid of
+0.005 cm bins, 100 batches, A_z/R_r/T_r/A(r,z) tallies) through the C ABI. Before timing, the same
+kernel is checked photon for photon against the oracle (the reference's identical-photon limits);
+a failing check aborts the benchmark. Prints one JSON line.
+
+roofline: the transport kernel is bound by vector ALU issue (not HBM, not the matrix cores).
+Its algorithmic work is 190 VALU lane-op slots per absorption event and 100 per boundary step
+(add/mul/fma/cmp/select/cvt/shift/xor/mul_hi = 1 slot, a transcendental = 4), counted from the
+reference's algorithm; the peak is 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-op/s.
+The kernel time is measured with HIP events on the kernel's own stream inside the engine.
+
+cpu_baseline: the reference's CPU engine (float64, all host cores), built from its sources by
+oracle/Makefile, timed on a bounded sample of the same workload on this machine's host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "photons/s at 3 layers; stacks/s surrogate vs batched MC at matched error"
+SLOTS_PER_EVENT = 190
+SLOTS_PER_BOUNDARY_STEP = 100
+PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9  # 3.93e13 lane-op/s
+
+WORKLOADS = {
+    "c2": ("three_layer_c2", 100_000_000, 100,
+           "three-layer stack (n 1.4/1.4/1.44; mua 4/0.5/0.1 /cm; mus 400/200/120 /cm; "
+           "g 0.8/0.85/0.9; 0.01/0.2/0.5 cm), 1e8 photons, 200x200 grid at 0.005 cm, 100 batches"),
+    "c1": ("slab_c1", 1_000_000, 100,
+           "single slab (n 1.37, mua 1, mus 100, g 0.9, 0.1 cm), 1e6 photons, 100x100 grid at "
+           "0.001 cm, 100 batches"),
+    "tissue": ("tissue_g09", 4_000_000, 100,
+               "semi-infinite tissue (n 1.37, mua 0.1, mus 100, g 0.9), 4e6 photons, 40x50 grid"),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gate(engine, spec: dict) -> None:
+    import oracle
+    from tissuemc_amd import cross_engine
+
+    n = 100_000 if spec["layers"][0]["d"] < 1e3 else 20_000
+    g = engine.run(spec, photons=n, batches=20, seed=97, arz=True)
+    o = oracle.run(spec, photons=n, batches=20, seed=97, precision="float32")
+    bad = [x.line() for x in cross_engine.compare_runs(o, g) if not x.passed]
+    if bad or g["anomalies"]:
+        raise SystemExit("bench: validation gate failed, not timing:\n" + "\n".join(bad))
+    log(f"bench: gate passed ({n} photons identical-photon check)")
+
+
+def cpu_baseline(mci: Path, full_photons: int, batches: int, target_s: float = 12.0) -> dict:
+    """The reference CPU engine on a bounded sample, all host cores; or the oracle if the
+    reference build is absent."""
+    import oracle
+
+    cores = os.cpu_count() or 1
+    sample = 200_000
+    if oracle.ref_available():
+        def timed(n: int) -> float:
+            with tempfile.TemporaryDirectory() as tmp:
+                t0 = time.perf_counter()
+                oracle.run_ref(mci, photons=n, batches=batches, seed=1, threads=0,
+                               out_json=Path(tmp) / "r.json")
+                return time.perf_counter() - t0
+        kind = "reference"
+    else:
+        spec = oracle.parse_mci_text(mci.read_text())[0]
+
+        def timed(n: int) -> float:
+            t0 = time.perf_counter()
+            oracle.run(spec, photons=n, batches=batches, seed=1, threads=0)
+            return time.perf_counter() - t0
+        kind = "port"
+    t = timed(sample)  # calibration, also the warm-up
+    n = max(batches, min(full_photons, int(sample * target_s / max(t, 1e-3))))
+    t = timed(n)
+    return {"value": n / t, "unit": "photons/s", "cores": min(cores, batches), "kind": kind,
+            "sample": f"{n} photons of the same workload (seed 1, {batches} batches), float64, "
+                      f"{t:.1f} s wall, {min(cores, batches)} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--no-arz", action="store_true", help="leave A(r,z) out of the tallies")
+    ap.add_argument("--no-gate", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        return 2
+
+    import tissuemc_amd.engine as engine
+
+    name, photons, batches, desc = WORKLOADS[args.workload]
+    mci = ROOT / "tissuemc_amd" / "configs" / f"{name}.mci"
+    spec = engine.parse_mci(path=mci)[0]
+    arz = not args.no_arz
+    if not args.no_gate:
+        gate(engine, spec)
+
+    for i in range(args.warmup):
+        engine.run(spec, photons=photons, batches=batches, seed=1, arz=arz)
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    results = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        results.append(engine.run(spec, photons=photons, batches=batches, seed=1 + i, arz=arz))
+        log(f"bench: step {i + 1}/{args.steps}: kernel {results[-1]['kernel_seconds'] * 1e3:.1f} ms")
+    wall = time.perf_counter() - t0
+
+    events = sum(r["events"] for r in results)
+    bsteps = sum(r["boundary_steps"] for r in results)
+    kernel_s = sum(r["kernel_seconds"] for r in results)
+    work = events * SLOTS_PER_EVENT + bsteps * SLOTS_PER_BOUNDARY_STEP
+    achieved = work / kernel_s
+    traffic = None
+    tfile = ROOT / "profiles" / f"transport_{args.workload}_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": photons * args.steps / wall,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "photons": photons, "batches": batches,
+                   "tallies": "rd,tt,absorbed,A_z,R_r,T_r" + (",A_rz" if arz else ""),
+                   "tally_type": "int64 fixed point 2^-32", "seeds": f"1..{args.steps}"},
+        "events_per_s": events / kernel_s,
+        "photon_steps_per_s": (events + bsteps) / kernel_s,
+        "kernel_ms_per_step": kernel_s / args.steps * 1e3,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_LANE_OPS / 1e12,
+                     "unit": "Tlane-op/s", "frac": achieved / PEAK_LANE_OPS, "traffic": traffic},
+        "device": results[0]["device"],
+        "launch": results[0]["launch"],
+    }
+    if not args.no_cpu:
+        log("bench: timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(mci, photons, batches)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

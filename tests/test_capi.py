@@ -1,0 +1,159 @@
+"""The C-ABI library on the CPU: it loads, exports every declared symbol, parses .mci input like
+the reference (tests/cpp/test_mci.cpp), and rejects bad runs before touching a device."""
+
+import re
+import subprocess
+
+import pytest
+
+import tissuemc_amd.engine as engine
+from conftest import CONFIGS, ROOT
+
+HEADER = ROOT / "include" / "tissuemc_hip.h"
+CLI = ROOT / "tissuemc_amd" / "bin" / "tissuemc"
+
+THREE_LAYER = """1.0 # file version
+1 # number of runs
+### Specify data
+comp.mco A # output filename, ASCII/Binary
+1000000 # No. of photons
+.01 .01 # dz, dr
+40 50 1 # No. of dz, dr & da.
+3 # No. of layers
+# n mua mus g d # One line for each layer
+1.0 # n for medium above.
+1.37 1 100 0.90 0.1 # layer 1
+1.37 1 10 0 0.1 # layer 2
+1.37 2 10 0.70 0.2 # layer 3
+1.0 # n for medium below.
+"""
+
+
+def declared_functions() -> list[str]:
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|double|void|const char \*)\s*(tmc_\w+)\s*\(", text,
+                                 flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    L = engine.load()
+    names = declared_functions()
+    assert len(names) >= 9
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) <= set(engine.EXPORTED) | {"tmc_hip_run"}
+    assert L.tmc_fixed_point_unit() == 2.0 ** -32
+
+
+def test_exported_symbols_are_plain_c():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(engine.LIB_PATH)], check=True,
+                         capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for name in declared_functions():
+        assert name in exported, name
+
+
+def mutate(old: str, new: str) -> str:
+    assert old in THREE_LAYER
+    return THREE_LAYER.replace(old, new, 1)
+
+
+def parse_error(text: str) -> str:
+    try:
+        engine.parse_mci(text)
+    except engine.ConfigError as e:
+        return str(e)
+    return ""
+
+
+def test_parses_the_manual_three_layer_example():
+    (r,) = engine.parse_mci(THREE_LAYER)
+    assert r["output_name"] == "comp.mco" and r["photons"] == 1_000_000
+    assert (r["dz"], r["dr"], r["nz"], r["nr"], r["na"]) == (0.01, 0.01, 40, 50, 1)
+    assert r["layers"][0] == {"n": 1.37, "mua": 1.0, "mus": 100.0, "g": 0.9, "d": 0.1}
+    assert r["layers"][2]["mua"] == 2.0 and r["layers"][2]["d"] == 0.2
+
+
+def test_parses_several_runs_and_scientific_notation():
+    text = "1.0\n2\n"
+    text += "a.mco A\n100\n0.1 0.1\n1 1 1\n1\n1.0\n1.0 1 9 0 1e8\n1.0\n"
+    text += "b.mco A\n200\n0.1 0.1\n1 1 1\n1\n1.5\n1.4 1 9 0.5 2E-1\n1.5\n"
+    a, b = engine.parse_mci(text)
+    assert a["layers"][0]["d"] == 1e8 and b["photons"] == 200 and b["n_above"] == 1.5
+    assert b["layers"][0]["d"] == pytest.approx(0.2)
+
+
+def test_errors_name_the_offending_line():
+    assert "line 1" in parse_error(mutate("1.0 # file version", "2.0 # file version"))
+    assert "line 4" in parse_error(mutate("comp.mco A", "comp.mco B"))
+    assert "line 5" in parse_error(mutate("1000000 #", "1000000.5 #"))
+    assert "line 12" in parse_error(mutate("1.37 1 10 0 0.1", "1.37 1 10 0"))
+    assert "line 11" in parse_error(mutate("1.37 1 100 0.90 0.1", "1.37 1 100 abc 0.1"))
+
+
+@pytest.mark.parametrize("old,new", [
+    ("1.37 1 100 0.90 0.1", "1.37 1 100 1.0 0.1"),   # g = 1
+    ("1.37 1 100 0.90 0.1", "1.37 -1 100 0.9 0.1"),  # mua < 0
+    ("1.37 1 100 0.90 0.1", "1.37 1 100 0.9 0"),     # d = 0
+    ("1.37 1 100 0.90 0.1", "0.9 1 100 0.9 0.1"),    # n < 1
+    ("1.37 1 100 0.90 0.1", "1.37 0 0 0 0.1"),       # clear layer
+    ("1000000 #", "0 #"),                            # no photons
+    ("3 # No. of layers", "17 # No. of layers"),     # too many layers
+])
+def test_rejects_invalid_and_unsupported_input(old, new):
+    assert parse_error(mutate(old, new))
+
+
+def test_rejects_truncated_and_over_long_files():
+    assert parse_error("")
+    assert parse_error("1.0\n1\ncomp.mco A\n")
+    assert parse_error(THREE_LAYER + "extra line\n")
+    assert parse_error(mutate("1 # number of runs", "2 # number of runs"))
+
+
+def test_missing_file_names_the_file():
+    with pytest.raises(engine.ConfigError, match="definitely_missing.mci"):
+        engine.parse_mci(path="/nonexistent/definitely_missing.mci")
+
+
+def test_bundled_inputs_parse():
+    for p in sorted(CONFIGS.glob("*.mci")):
+        assert len(engine.parse_mci(path=p)) == 1
+
+
+def test_bad_runs_are_rejected_before_any_device_work():
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    with pytest.raises(engine.ConfigError, match="batches must be >= 1"):
+        engine.run(spec, photons=10, batches=0)
+    with pytest.raises(engine.ConfigError, match="cannot exceed"):
+        engine.run(spec, photons=10, batches=11)
+    with pytest.raises(engine.ConfigError, match="layer 1: g must be"):
+        engine.run(dict(spec, layers=[dict(spec["layers"][0], g=-1.0)]), photons=10, batches=1)
+    with pytest.raises(engine.ConfigError, match="ambient"):
+        engine.validate_spec(dict(spec, n_above=0.5))
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if engine.available():
+        pytest.skip("a GPU is present")
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    with pytest.raises(engine.DeviceError, match="no usable HIP device"):
+        engine.run(spec, photons=100, batches=1)
+
+
+def cli(*args):
+    return subprocess.run([str(CLI), *args], capture_output=True, text=True)
+
+
+def test_cli_surface():
+    assert cli("--version").returncode == 0
+    assert cli().returncode == 2
+    assert cli("run").returncode == 2
+    r = cli("run", str(CONFIGS / "v1_slab.mci"), "--bogus")
+    assert r.returncode == 2 and "unknown option" in r.stderr
+    r = cli("run", str(CONFIGS / "v1_slab.mci"), "--batches", "-3")
+    assert r.returncode == 2
+    r = cli("run", str(CONFIGS / "v1_slab.mci"), "--engine", "cpu")
+    assert r.returncode == 2 and "oracle" in r.stderr
+    r = cli("run", "/nonexistent/x.mci")
+    assert r.returncode == 1 and "tissuemc: error:" in r.stderr and "x.mci" in r.stderr

@@ -1,0 +1,196 @@
+"""Parity of the HIP transport kernel with the oracle, through the C ABI.
+
+- Random streams: bit-exact with the oracle's Philox (hence with the reference's PhotonRng).
+- Identical photons: the GPU (float32, AMD transcendentals) against the oracle's float32 path,
+  within the reference's cross-engine limits (tissuemc_amd.cross_engine), on every workload.
+- Exact integer identities of the fixed-point tallies, per batch.
+- Bit-identical tallies for a seed, whatever the launch shape or the A(r,z) tiling.
+- Every scalar and profile bin within 3 combined standard errors of the float64 oracle on the
+  same photons; published values within 3 sigma (tissuemc_amd.validate).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_spec
+from tissuemc_amd import cross_engine, stats, validate
+
+pytestmark = pytest.mark.gpu
+
+# (workload, photons for the identical-photon comparison, batches). The reference fixed its
+# limits for 2e6 photons in 50 batches (4e4 per batch); one photon that takes another path moves
+# a batch of n photons by about 1/n, so batches here hold 1e4 photons or more.
+WORKLOADS = [
+    ("v1_slab", 400_000, 10),
+    ("v2_semi_infinite", 200_000, 10),
+    ("o1_semi_infinite_matched", 200_000, 10),
+    ("v3_three_layer", 800_000, 20),
+    ("tissue_g09", 40_000, 4),
+    ("slab_c1", 400_000, 10),
+    ("three_layer_c2", 1_000_000, 10),
+]
+
+
+@pytest.mark.parametrize("seed,pid,word3", [(0, 0, 0), (1, 7, 0), (42, 1_000_003, 0),
+                                            (0xDEADBEEFCAFEF00D, 5, 0), (7, 0x1_0000_0005, 0),
+                                            (2**64 - 1, 2**64 - 1, 0), (3, 11, 99)])
+def test_device_philox_stream_is_bit_exact(gpu, seed, pid, word3):
+    n = 4099  # crosses ~1000 block boundaries and ends mid-block
+    assert np.array_equal(gpu.philox_stream(seed, pid, n, word3), oracle.stream_u32(seed, pid, n, word3))
+
+
+@pytest.mark.parametrize("case,photons,batches", WORKLOADS, ids=[w[0] for w in WORKLOADS])
+def test_identical_photons_within_reference_limits(gpu, case, photons, batches):
+    spec = load_spec(case)
+    g = gpu.run(spec, photons=photons, batches=batches, seed=11)
+    o = oracle.run(spec, photons=photons, batches=batches, seed=11, precision="float32")
+    gaps = cross_engine.compare_runs(o, g)
+    failed = [x.line() for x in gaps if not x.passed]
+    assert not failed, "\n".join(failed)
+    assert g["anomalies"] == 0
+
+
+@pytest.mark.parametrize("case", ["v3_three_layer", "three_layer_c2", "slab_c1"])
+def test_fixed_point_identities_are_exact(gpu, case):
+    spec = load_spec(case)
+    g = gpu.run(spec, photons=200_000, batches=7, seed=5, arz=True)
+    f = g["fixed"]
+    for b in range(7):
+        assert int(f["az"][b].sum()) == int(f["absorbed"][b].sum())   # A_z covers all absorption
+        assert int(f["rr"][b].sum()) == int(f["rd"][b])                # R_r covers R_d
+        assert int(f["tr"][b].sum()) == int(f["tt"][b])                # T_r covers T_t
+    # A(r,z) has no batch dimension: its rows sum to A_z summed over batches.
+    assert np.array_equal(f["arz"].sum(axis=1, dtype=np.uint64), f["az"].sum(axis=0, dtype=np.uint64))
+    assert [b["photons"] for b in g["batches"]] == [28572] * 3 + [28571] * 4
+
+
+def test_bit_identical_for_any_launch_shape(gpu):
+    spec = load_spec("three_layer_c2")
+    kw = dict(photons=150_000, batches=10, seed=3, arz=True)
+    ref = gpu.run(spec, **kw)
+    variants = [dict(), dict(groups=37, waves_per_group=5), dict(waves_per_group=1, chunk=64),
+                dict(chunk=4000), dict(arz_tile=(0, 0)), dict(arz_tile=(200, 1)),
+                dict(arz_tile=(13, 57), groups=300)]
+    for v in variants:
+        r = gpu.run(spec, **kw, **v)
+        for key in ("rd", "tt", "absorbed", "az", "rr", "tr", "arz"):
+            assert np.array_equal(r["fixed"][key], ref["fixed"][key]), (v, key)
+        assert [b["events"] for b in r["batches"]] == [b["events"] for b in ref["batches"]]
+
+
+def test_profiles_off_diagnostic_leaves_totals_unchanged(gpu):
+    spec = load_spec("v3_three_layer")
+    a = gpu.run(spec, photons=100_000, batches=10, seed=8)
+    b = gpu.run(spec, photons=100_000, batches=10, seed=8, profiles=False)
+    for key in ("rd", "tt", "absorbed"):
+        assert np.array_equal(a["fixed"][key], b["fixed"][key])
+    assert a["events"] == b["events"]
+    assert not b["fixed"]["az"].any() and not b["fixed"]["rr"].any()
+
+
+@pytest.mark.parametrize("case", ["v3_three_layer", "slab_c1", "three_layer_c2"])
+def test_every_scalar_and_bin_within_3_sigma_of_float64_oracle(gpu, case):
+    """Same input, same seed: per batch the GPU and the float64 oracle follow the same photons,
+    so every bin must sit well inside 3 combined standard errors."""
+    spec = load_spec(case)
+    n, B = (200_000, 20) if case != "three_layer_c2" else (60_000, 20)
+    g = gpu.run(spec, photons=n, batches=B, seed=21)
+    o = oracle.run(spec, photons=n, batches=B, seed=21, precision="float64")
+    ph = [b["photons"] for b in o["batches"]]
+    for key in ("rd", "tt"):
+        eo = stats.batch_estimate([b[key] for b in o["batches"]], ph)
+        eg = stats.batch_estimate([b[key] for b in g["batches"]], ph)
+        assert stats.compare(eg.mean, eg.se, eo.mean, eo.se).passed, key
+    for gk, ok in (("az_b", "az_b"), ("rr_b", "rr_b"), ("tr_b", "tr_b")):
+        mg, sg = stats.profile_estimate(g[gk].astype(np.float64) * 2.0 ** -32, ph)
+        mo, so = stats.profile_estimate(o[ok], ph)
+        z = stats.profile_z(mg, sg, mo, so)
+        assert np.all(np.abs(z) <= 3.0), (gk, float(np.max(np.abs(z))))
+
+
+def test_independent_seeds_agree_statistically(gpu):
+    """Different photons (GPU seed 2 at 4e6, oracle seed 1 at 4e5): the per-bin z-scores of the
+    depth and radius profiles behave like standard normals (few beyond 3, chi-square near the
+    number of bins)."""
+    spec = load_spec("v3_three_layer")
+    g = gpu.run(spec, photons=4_000_000, batches=40, seed=2)
+    o = oracle.run(spec, photons=400_000, batches=40, seed=1, precision="float64")
+    ph_g = [b["photons"] for b in g["batches"]]
+    ph_o = [b["photons"] for b in o["batches"]]
+    for gk in ("az_b", "rr_b", "tr_b"):
+        mg, sg = stats.profile_estimate(g[gk].astype(np.float64) * 2.0 ** -32, ph_g)
+        mo, so = stats.profile_estimate(o[gk], ph_o)
+        keep = (sg > 0) & (so > 0)
+        z = stats.profile_z(mg, sg, mo, so)[keep]
+        assert np.mean(np.abs(z) > 3) <= 0.05, gk
+        assert np.sum(z ** 2) < len(z) + 5 * math.sqrt(2 * len(z)), gk
+
+
+def test_weight_one_exits_are_exact(gpu):
+    """A non-absorbing matched slab: every packet leaves with weight exactly 1, so R_d + T_t is
+    exactly photons * 2^32 units, and each batch's tallies are whole photons."""
+    spec = {"photons": 50_000, "dz": 0.01, "dr": 0.01, "nz": 5, "nr": 5, "na": 1, "n_above": 1.0,
+            "n_below": 1.0, "layers": [{"n": 1.0, "mua": 0.0, "mus": 50.0, "g": 0.5, "d": 0.05}]}
+    g = gpu.run(spec, batches=5, seed=4)
+    f = g["fixed"]
+    assert int(f["rd"].sum()) + int(f["tt"].sum()) == 50_000 * 2**32
+    assert all(int(v) % 2**32 == 0 for v in list(f["rd"]) + list(f["tt"]))
+    assert g["events"] > 0 and int(f["absorbed"].sum()) == 0
+
+
+def test_edge_shapes(gpu):
+    spec = load_spec("v3_three_layer")
+    one = gpu.run(spec, photons=5000, batches=1, seed=4)
+    assert len(one["batches"]) == 1 and one["batches"][0]["photons"] == 5000
+    each = gpu.run(spec, photons=40, batches=40, seed=5)
+    assert [b["photons"] for b in each["batches"]] == [1] * 40
+    o = oracle.run(spec, photons=40, batches=40, seed=5, precision="float32")
+    assert sum(cross_engine._sums(o, "rd")) == pytest.approx(sum(cross_engine._sums(each, "rd")),
+                                                             rel=5e-3, abs=1e-6)
+    few = gpu.run(spec, photons=3, batches=1, seed=6)
+    assert few["photons"] == 3
+
+
+def test_sixteen_layers(gpu):
+    layers = [{"n": 1.33 + 0.01 * i, "mua": 0.5 + 0.1 * i, "mus": 50 + 5 * i, "g": 0.7,
+               "d": 0.01} for i in range(16)]
+    spec = {"photons": 100_000, "dz": 0.01, "dr": 0.01, "nz": 20, "nr": 20, "na": 1,
+            "n_above": 1.0, "n_below": 1.2, "layers": layers}
+    g = gpu.run(spec, batches=10, seed=2)
+    o = oracle.run(spec, batches=10, seed=2, precision="float32")
+    failed = [x.line() for x in cross_engine.compare_runs(o, g) if not x.passed]
+    assert not failed, "\n".join(failed)
+
+
+def test_step_limit_is_reported_as_anomaly(gpu):
+    spec = load_spec("tissue_g09")
+    g = gpu.run(spec, photons=2000, batches=2, seed=1, step_limit=50)
+    assert g["anomalies"] > 0
+
+
+def test_errors_through_the_c_abi(gpu):
+    spec = load_spec("v1_slab")
+    with pytest.raises(gpu.ConfigError, match="batches must be >= 1"):
+        gpu.run(spec, photons=10, batches=0)
+    with pytest.raises(gpu.ConfigError, match="cannot exceed"):
+        gpu.run(spec, photons=10, batches=11)
+    with pytest.raises(gpu.ConfigError, match="does not exist"):
+        gpu.run(spec, photons=10, batches=1, device=99)
+    with pytest.raises(gpu.ConfigError, match="g must be"):
+        gpu.run(dict(spec, layers=[dict(spec["layers"][0], g=1.0)]), photons=10, batches=1)
+    with pytest.raises(gpu.ConfigError, match="LDS"):
+        gpu.run(dict(spec, nz=1_000_000), photons=10, batches=1)
+
+
+@pytest.mark.parametrize("case", list(validate.references()))
+@pytest.mark.parametrize("seed", [1, 2])
+def test_published_values_within_3_sigma(gpu, case, seed):
+    ref = validate.references()[case]
+    spec = load_spec(case)
+    run = gpu.run(spec, photons=ref["photons"], batches=50, seed=seed)
+    checks = validate.validate_run(case, ref, run, seed)
+    failed = [c.line() for c in checks if not c.comparison.passed]
+    assert not failed, "\n".join(failed)

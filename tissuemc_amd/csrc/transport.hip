@@ -13,14 +13,20 @@
 //   When its batch has no ids left, the wave drains its live photons, adds its LDS tally slot into
 //   the batch's HBM arrays and moves to the next batch that still has ids.
 // * Tallies are integers in units of 2^-32 (a float32 weight >= 2^-8 converts exactly). Each
-//   wave has a private LDS slot for its current batch (scalars, A_z, R_r, T_r); the workgroup
-//   shares an LDS tile of A(r,z) for the bins nearest the source, the rest go to HBM atomics.
-//   Integer sums do not depend on order, so a seed gives bit-identical tallies on every run.
+//   wave has a private LDS slot for its current batch (scalars and A_z, scored at every event).
+//   R_r and T_r are scored once per photon and go straight to the batch's HBM arrays. The
+//   workgroup shares an LDS tile of A(r,z) as 32-bit counters: an add that wraps a counter
+//   carries 2^32 into the HBM bin (the LDS atomic returns the old value, so the carry is exact),
+//   which halves the tile's footprint so that it covers most of the grid; bins outside the tile
+//   go to HBM atomics. Integer sums do not depend on order, so a seed gives bit-identical
+//   tallies on every run.
 // * Random words: each lane buffers two Philox blocks (8 words). At the top of every step it
 //   refills if fewer than 5 words are left, so the at-most-4 draws of a step never refill in the
 //   middle of divergent code; the words of the step are rotated into r0..r3 once.
 // * Math uses the gfx950 transcendental units directly: v_log_f32 (log2), v_sin_f32/v_cos_f32
 //   (argument in revolutions, so the azimuth 2*pi*xi needs no multiply), v_sqrt, v_rsq, v_rcp.
+//   The file is compiled without automatic FMA contraction and every fused multiply-add is
+//   written out, so that all instantiations of the kernel follow bit-identical photon paths.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,8 +78,8 @@ struct KernelArgs {
     uint64_t *next;        // [B] ids claimed so far in each batch
     uint64_t *rd, *tt, *absorbed, *events, *anomalies, *bsteps, *az, *rr, *tr, *arz;
     int32_t slot_words;  // u64 words per wave slot
-    int32_t off_az, off_rr, off_tr;
-    int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] at word tile_off
+    int32_t off_az;
+    int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] u32 at word tile_off
     int32_t layer_off;                 // word offset of the layer table (floats) in LDS
     uint32_t chunk;
     uint32_t step_limit;
@@ -140,9 +146,7 @@ __device__ void flush_slot(const KernelArgs &a, uint64_t *slot, uint64_t b, int 
         else if (j == kSlotAnomalies) dst = a.anomalies + b;
         else if (j == kSlotBoundary) dst = a.bsteps + b;
         else if (j < kSlotAbs + L) dst = a.absorbed + b * L + (j - kSlotAbs);
-        else if (j < a.off_rr) dst = a.az + b * a.nz + (j - a.off_az);
-        else if (j < a.off_tr) dst = a.rr + b * a.nr + (j - a.off_rr);
-        else dst = a.tr + b * a.nr + (j - a.off_tr);
+        else dst = a.az + b * a.nz + (j - a.off_az);
         hbm_add(dst, v);
     }
 }
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     const int waves = static_cast<int>(blockDim.x >> 6);
 
     // Clear the wave slots and the A(r,z) tile; copy the layer table into LDS.
-    const int words = waves * a.slot_words + (kArz ? a.tile_z * a.tile_r : 0);
+    const int words = waves * a.slot_words + (kArz ? (a.tile_z * a.tile_r + 1) / 2 : 0);
     for (int i = static_cast<int>(threadIdx.x); i < words; i += static_cast<int>(blockDim.x))
         lds[i] = 0;
     float *ltab = reinterpret_cast<float *>(lds + a.layer_off);
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     __syncthreads();
 
     uint64_t *slot = lds + wid * a.slot_words;
-    uint64_t *tile = lds + a.tile_off;
+    uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
     const int last_layer = a.num_layers - 1;
 
     // ---- wave-uniform scheduling state
@@ -318,17 +322,17 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 } else if (ca1 < 1e-6f) {
                     r = 1.0f;
                 } else {
-                    const float sa1 = __builtin_amdgcn_sqrtf(1.0f - ca1 * ca1);
+                    const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
                     const float sa2 = ni * sa1 * __builtin_amdgcn_rcpf(nt);
                     if (sa2 >= 1.0f) {
                         r = 1.0f;
                     } else {
-                        ca2 = __builtin_amdgcn_sqrtf(1.0f - sa2 * sa2);
-                        const float cap = ca1 * ca2 - sa1 * sa2;
-                        const float cam = ca1 * ca2 + sa1 * sa2;
-                        const float sap = sa1 * ca2 + ca1 * sa2;
-                        const float sam = sa1 * ca2 - ca1 * sa2;
-                        r = 0.5f * sam * sam * (cam * cam + cap * cap) *
+                        ca2 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-sa2, sa2, 1.0f));
+                        const float cap = __builtin_fmaf(ca1, ca2, -(sa1 * sa2));
+                        const float cam = __builtin_fmaf(ca1, ca2, sa1 * sa2);
+                        const float sap = __builtin_fmaf(sa1, ca2, ca1 * sa2);
+                        const float sam = __builtin_fmaf(sa1, ca2, -(ca1 * sa2));
+                        r = 0.5f * sam * sam * __builtin_fmaf(cam, cam, cap * cap) *
                             __builtin_amdgcn_rcpf(sap * sap * cam * cam);
                     }
                 }
@@ -349,7 +353,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                     if (kProfiles) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                         const int ir = bin_of(rr, a.inv_dr, a.nr);
-                        lds_add(slot + (exit_top ? a.off_rr : a.off_tr) + ir, qw);
+                        hbm_add((exit_top ? a.rr : a.tr) + b * a.nr + ir, qw);
                     }
                     died = true;
                 } else {
@@ -379,10 +383,17 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 if (kArz) {
                     const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                     const int ir = bin_of(rr, a.inv_dr, a.nr);
-                    if (iz < a.tile_z && ir < a.tile_r)
-                        lds_add(tile + iz * a.tile_r + ir, q);
-                    else
-                        hbm_add(a.arz + static_cast<uint64_t>(iz) * a.nr + ir, q);
+                    uint64_t *hbm_bin = a.arz + static_cast<uint64_t>(iz) * a.nr + ir;
+                    if (iz < a.tile_z && ir < a.tile_r) {
+                        const uint32_t lo = static_cast<uint32_t>(q);
+                        const uint32_t old = __hip_atomic_fetch_add(
+                            tile + iz * a.tile_r + ir, lo, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint64_t carry = (q >> 32) + (old + lo < old ? 1u : 0u);
+                        if (carry != 0) hbm_add(hbm_bin, carry << 32);
+                    } else {
+                        hbm_add(hbm_bin, q);
+                    }
                 }
             }
             // spin (physics.hpp:116-144) with the HG sampler (physics.hpp:101-113)
@@ -391,13 +402,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             c += 2;
             float cost;
             if (P[kHgC] == 0.0f) {
-                cost = 2.0f * xi_hg - 1.0f;
+                cost = __builtin_fmaf(2.0f, xi_hg, -1.0f);
             } else {
                 const float t = P[kHgA] * __builtin_amdgcn_rcpf(__builtin_fmaf(P[kHgC], xi_hg, P[kHgB]));
-                cost = (P[kHgD] - t * t) * P[kHgE];
+                cost = __builtin_fmaf(-t, t, P[kHgD]) * P[kHgE];
                 cost = __builtin_fminf(__builtin_fmaxf(cost, -1.0f), 1.0f);
             }
-            const float s2 = 1.0f - cost * cost;
+            const float s2 = __builtin_fmaf(-cost, cost, 1.0f);
             const float sint = s2 > 0.0f ? __builtin_amdgcn_sqrtf(s2) : 0.0f;
             const float cosp = __builtin_amdgcn_cosf(xi_az);  // cos(2 pi xi)
             const float sinp = __builtin_amdgcn_sinf(xi_az);  // sin(2 pi xi)
@@ -407,13 +418,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 ny = sint * sinp;
                 nz_ = uz >= 0.0f ? cost : -cost;
             } else {
-                const float t2 = 1.0f - uz * uz;
+                const float t2 = __builtin_fmaf(-uz, uz, 1.0f);
                 const float rt = __builtin_amdgcn_rsqf(t2);
                 const float temp = t2 * rt;
                 const float sr = sint * rt;
-                nx = sr * (ux * uz * cosp - uy * sinp) + ux * cost;
-                ny = sr * (uy * uz * cosp + ux * sinp) + uy * cost;
-                nz_ = -sint * cosp * temp + uz * cost;
+                nx = __builtin_fmaf(sr, __builtin_fmaf(ux * uz, cosp, -(uy * sinp)), ux * cost);
+                ny = __builtin_fmaf(sr, __builtin_fmaf(uy * uz, cosp, ux * sinp), uy * cost);
+                nz_ = __builtin_fmaf(-(sint * cosp), temp, uz * cost);
             }
             const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz_ * nz_)));
             ux = nx * inv;
@@ -607,9 +618,7 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
                                   : 1000000000u;
 
     a.off_az = kSlotAbs + L;
-    a.off_rr = a.off_az + (profiles ? nz : 0);
-    a.off_tr = a.off_rr + (profiles ? nr : 0);
-    a.slot_words = a.off_tr + (profiles ? nr : 0);
+    a.slot_words = a.off_az + (profiles ? nz : 0);
 
     const size_t lds_cap = dev.lds_per_block;
     const size_t layer_bytes = static_cast<size_t>(L) * kLayerFields * sizeof(float);
@@ -624,23 +633,31 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
                               std::to_string(lds_cap) + " per workgroup)");
     int tile_z = 0, tile_r = 0;
     if (want_arz) {
-        const size_t free_words = (lds_cap - slots_bytes(wpg) - layer_bytes) / 8;
+        const size_t free_bins = (lds_cap - slots_bytes(wpg) - layer_bytes) / 4;  // u32 counters
         if (opt.arz_tile_z >= 0 && opt.arz_tile_r >= 0) {
             tile_z = std::min(opt.arz_tile_z, nz);
             tile_r = std::min(opt.arz_tile_r, nr);
-            if (static_cast<size_t>(tile_z) * tile_r > free_words)
+            if (static_cast<size_t>(tile_z) * tile_r > free_bins)
                 throw EngineError(TMC_ERR_CONFIG, "A(r,z) tile does not fit in LDS");
         } else {
-            // Bins nearest the source first: a square-ish region, full width when it fits.
-            tile_r = static_cast<int>(std::min<size_t>(nr, std::max<size_t>(1, static_cast<size_t>(std::sqrt(static_cast<double>(free_words))))));
-            tile_z = static_cast<int>(std::min<size_t>(nz, free_words / std::max(tile_r, 1)));
-            if (tile_z == nz) tile_r = static_cast<int>(std::min<size_t>(nr, free_words / std::max(nz, 1)));
+            // Depth bins below the stack never receive weight; cover every reachable depth and
+            // as many radii from the axis outward as fit (deposits thin out with r).
+            double depth = 0.0;
+            for (int i = 0; i < L; ++i) depth += spec.layers[i].d;
+            const double reach = std::floor(depth / spec.dz) + 2.0;
+            const int nz_eff = reach < static_cast<double>(nz) ? static_cast<int>(reach) : nz;
+            tile_z = nz_eff;
+            tile_r = static_cast<int>(std::min<size_t>(nr, free_bins / std::max(tile_z, 1)));
+            if (tile_r < std::min(nr, 32)) {
+                tile_r = static_cast<int>(std::min<size_t>(nr, std::max<size_t>(1, static_cast<size_t>(std::sqrt(static_cast<double>(free_bins))))));
+                tile_z = static_cast<int>(std::min<size_t>(nz_eff, free_bins / std::max(tile_r, 1)));
+            }
         }
     }
     a.tile_z = tile_z;
     a.tile_r = tile_r;
     a.tile_off = wpg * a.slot_words;
-    a.layer_off = a.tile_off + tile_z * tile_r;
+    a.layer_off = a.tile_off + (tile_z * tile_r + 1) / 2;
     const size_t lds_bytes = static_cast<size_t>(a.layer_off) * 8 + layer_bytes;
 
     const int groups = opt.groups > 0 ? opt.groups : dev.cus;
