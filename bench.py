@@ -96,6 +96,32 @@ def cpu_baseline(mci: Path, full_photons: int, batches: int, target_s: float = 1
                       f"{t:.1f} s wall, {min(cores, batches)} threads"}
 
 
+def bench_batched(engine, steps: int, warmup: int) -> dict:
+    """The batched workload: 4096 stacks (1-5 layers, budgets log-uniform 1e4-1e6), 10 batches per
+    stack, 100 x 100 grid at 0.01 cm, one graph-replayed persistent launch per step."""
+    stacks = engine.generate_stacks(2026, 0, 4096)
+    photons = sum(int(stacks[i].photons) for i in range(4096))
+    with engine.BatchedPlan(4096, batches_per_stack=10, dz=0.01, dr=0.01, nz=100, nr=100,
+                            photons_hint=photons) as plan:
+        for _ in range(warmup):
+            plan.execute(stacks, seed=1)
+        t0 = time.perf_counter()
+        res = [plan.execute(stacks, seed=1 + i) for i in range(steps)]
+        wall = time.perf_counter() - t0
+    kernel = sum(r["kernel_seconds"] for r in res)
+    events = sum(int(r["events"].sum()) for r in res)
+    bsteps = sum(int(r["boundary_steps"].sum()) for r in res)
+    log(f"bench: batched {steps} steps, kernel {kernel / steps * 1e3:.1f} ms per step")
+    return {"workload": "4096 stacks, 1-5 layers, budgets log-uniform 1e4-1e6 (generator seed "
+                        "2026), 10 batches per stack, 100x100 grid at 0.01 cm",
+            "stacks_per_s": 4096 * steps / wall, "photons_per_s": photons * steps / wall,
+            "photons_per_step": photons, "ms_per_step": wall / steps * 1e3,
+            "kernel_ms_per_step": kernel / steps * 1e3,
+            "roofline_frac": (events * SLOTS_PER_EVENT + bsteps * SLOTS_PER_BOUNDARY_STEP)
+                             / kernel / PEAK_LANE_OPS,
+            "launch": res[0]["launch"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +131,7 @@ def main() -> int:
     ap.add_argument("--no-arz", action="store_true", help="leave A(r,z) out of the tallies")
     ap.add_argument("--no-gate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-stack workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -163,6 +190,8 @@ def main() -> int:
         "device": results[0]["device"],
         "launch": results[0]["launch"],
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(engine, steps=2, warmup=1)
     if not args.no_cpu:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mci, photons, batches)

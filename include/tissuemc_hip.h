@@ -44,6 +44,7 @@ enum {
 /* Option flags (tmc_opts.flags). */
 #define TMC_FLAG_ARZ 1u           /* also tally A(r, z) into tmc_out.arz */
 #define TMC_FLAG_NO_PROFILES 2u   /* diagnostic: skip the A_z/R_r/T_r/A_rz profiles (zeros) */
+#define TMC_FLAG_NO_GRAPH 4u      /* batched runs: launch directly instead of replaying a graph */
 
 typedef struct tmc_layer {
     double n;   /* refractive index */
@@ -98,6 +99,84 @@ typedef struct tmc_out {
     char device[96];          /* device name and architecture */
     int32_t groups, waves_per_group, lds_bytes, chunk, arz_tile_z, arz_tile_r;
 } tmc_out;
+
+/* ---------------------------------------------------------------- batched multi-stack runs
+ *
+ * No reference counterpart (the reference simulates one stack per process). Many independent
+ * layer stacks, each with its own photon budget, run in one persistent launch; every stack is cut
+ * into batches_per_stack batches (the reference's BatchPlan over its budget) for standard errors.
+ * Stack s uses the Philox streams (seed, photon id, word3 = s): its results equal, bit for bit,
+ * a single run of the same stack with tmc_opts.stream = s and the same seed, batches and grid. */
+
+typedef struct tmc_stack {
+    int32_t num_layers;
+    double n_above, n_below;
+    tmc_layer layers[TMC_MAX_LAYERS];
+    uint64_t photons; /* this stack's photon budget */
+} tmc_stack;
+
+typedef struct tmc_batched_opts {
+    uint64_t seed;
+    int32_t batches_per_stack; /* >= 1 and <= every stack's budget */
+    int32_t device;
+    double dz, dr;             /* one grid for every stack */
+    int32_t nz, nr;
+    uint32_t flags;            /* TMC_FLAG_NO_PROFILES, TMC_FLAG_NO_GRAPH */
+    int32_t waves_per_group, groups, chunk; /* 0 = automatic */
+    uint64_t step_limit;       /* 0 = 1e9 */
+    uint64_t photons_hint;     /* expected photons per execute (sizes the claim chunk); 0 = auto */
+} tmc_batched_opts;
+
+/* Results of a batched run; S = stacks, Bs = batches_per_stack. Caller-owned buffers. */
+typedef struct tmc_batched_out {
+    uint64_t *rd, *tt;        /* [S*Bs]   row s*Bs + b */
+    uint64_t *absorbed;       /* [S*Bs*TMC_MAX_LAYERS] */
+    uint64_t *events, *anomalies, *boundary_steps; /* [S*Bs] */
+    uint64_t *az;             /* [S*nz] all batches of a stack */
+    uint64_t *rr, *tr;        /* [S*nr] */
+    double kernel_seconds;    /* clear + transport (HIP events) */
+    double wall_seconds;
+    char device[96];
+    int32_t groups, waves_per_group, lds_bytes, chunk;
+} tmc_batched_out;
+
+typedef struct tmc_batched_plan tmc_batched_plan; /* opaque */
+
+/* A resident plan for repeated batched runs of n_stacks stacks: device buffers are allocated
+ * once and the clear + transport launches are captured in a HIP graph on the first execute and
+ * replayed afterwards. */
+int tmc_batched_create(int32_t n_stacks, const tmc_batched_opts *opts, tmc_batched_plan **plan);
+int tmc_batched_execute(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
+                        uint64_t seed, tmc_batched_out *out);
+void tmc_batched_destroy(tmc_batched_plan *plan);
+
+/* One-shot form: create, execute once, destroy. */
+int tmc_hip_run_batched(const tmc_stack *stacks, int32_t n_stacks, const tmc_batched_opts *opts,
+                        tmc_batched_out *out);
+
+/* Distribution of random layer stacks. Log-uniform: mua, mus, d, photons; uniform: g, n; the
+ * layer count is uniform in [min_layers, max_layers]; the ambient index is fixed. */
+typedef struct tmc_stack_dist {
+    int32_t min_layers, max_layers;
+    double mua_lo, mua_hi;         /* [1/cm] */
+    double mus_lo, mus_hi;         /* [1/cm] */
+    double g_lo, g_hi;
+    double n_lo, n_hi;
+    double d_lo, d_hi;             /* [cm] */
+    double photons_lo, photons_hi; /* photon budget per stack */
+    double n_ambient;
+} tmc_stack_dist;
+
+/* The distribution of the batched workload: 1-5 layers, mua 0.01-10 /cm, mus 10-500 /cm,
+ * g 0.6-0.95, n 1.33-1.5, d 0.005-0.5 cm, budgets 1e4-1e6 photons, ambient 1.0. */
+void tmc_default_stack_dist(tmc_stack_dist *dist);
+
+/* Stacks first_id .. first_id + count - 1 of the sequence keyed by seed. Stack k depends only on
+ * (seed, k): field blocks of Philox4x32-10 with counter {k_lo, k_hi, block, 0x53544B31} and key
+ * seed, each word mapped to (x + 0.5) * 2^-32. Block 0 gives the layer count (word 0) and budget
+ * (word 1); block 1 + i gives layer i's mua, mus, g, n (words 0-3); block 17 + i its thickness. */
+int tmc_generate_stacks(uint64_t seed, uint64_t first_id, int32_t count, const tmc_stack_dist *dist,
+                        tmc_stack *out);
 
 /* 1 if a HIP device is present and usable, else 0. */
 int tmc_hip_available(void);

@@ -258,6 +258,53 @@ int tmc_parse_mci_file(const char *path, tmc_spec *runs, int32_t max_runs, int32
     });
 }
 
+struct tmc_batched_plan {
+    tmc::BatchedPlan *impl;
+};
+
+int tmc_batched_create(int32_t n_stacks, const tmc_batched_opts *opts, tmc_batched_plan **plan) {
+    return guarded([&] {
+        if (!opts || !plan) config_fail("opts and plan must not be NULL");
+        *plan = nullptr;
+        auto *h = new tmc_batched_plan{nullptr};
+        try {
+            h->impl = tmc::batched_create(n_stacks, *opts);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *plan = h;
+    });
+}
+
+int tmc_batched_execute(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
+                        uint64_t seed, tmc_batched_out *out) {
+    return guarded([&] {
+        if (!plan || !plan->impl || !stacks || !out) config_fail("plan, stacks and out must not be NULL");
+        if (!out->rd || !out->tt || !out->absorbed || !out->events || !out->anomalies)
+            config_fail("tmc_batched_out: rd, tt, absorbed, events and anomalies are required");
+        tmc::batched_execute(*plan->impl, stacks, n_stacks, seed, out);
+    });
+}
+
+void tmc_batched_destroy(tmc_batched_plan *plan) {
+    if (!plan) return;
+    tmc::batched_destroy(plan->impl);
+    delete plan;
+}
+
+int tmc_hip_run_batched(const tmc_stack *stacks, int32_t n_stacks, const tmc_batched_opts *opts,
+                        tmc_batched_out *out) {
+    tmc_batched_plan *plan = nullptr;
+    int rc = tmc_batched_create(n_stacks, opts, &plan);
+    if (rc != TMC_OK) return rc;
+    rc = tmc_batched_execute(plan, stacks, n_stacks, opts->seed, out);
+    const std::string err = g_last_error;
+    tmc_batched_destroy(plan);
+    g_last_error = err;
+    return rc;
+}
+
 int tmc_hip_philox_stream(uint64_t seed, uint64_t photon_id, uint32_t word3, int64_t n,
                           uint32_t *out, int32_t device) {
     return guarded([&] {

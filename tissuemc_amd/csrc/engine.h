@@ -1,6 +1,7 @@
 // Internal C++ interface between the C ABI (capi.cpp) and the HIP engine (transport.hip).
 #pragma once
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -26,6 +27,19 @@ void validate_spec(const tmc_spec &spec);
 
 // Runs the transport kernel for one run and fills `out` (capi.cpp checks pointers first).
 void run_transport(const tmc_spec &spec, const tmc_opts &opts, tmc_out *out);
+
+// Launch-shape choices shared by single and batched runs (0 / -1 = automatic).
+struct tmc_tuning {
+    int waves_per_group = 0, groups = 0, chunk = 0, arz_tile_z = -1, arz_tile_r = -1;
+    uint64_t step_limit = 0;
+};
+
+// Batched runs (opaque to callers of the C ABI).
+struct BatchedPlan;
+BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &opts);
+void batched_execute(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, uint64_t seed,
+                     tmc_batched_out *out);
+void batched_destroy(BatchedPlan *plan);
 
 // Device-side Philox stream, for the parity hook.
 void philox_stream(uint64_t seed, uint64_t id, uint32_t word3, int64_t n, uint32_t *out, int device);

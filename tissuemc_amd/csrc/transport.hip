@@ -7,19 +7,24 @@
 // bins clamp at both ends, a finite weight and a unit direction are checked at every score).
 // How it computes it is this file's own design:
 //
-// * One workgroup per compute unit, W waves each. A wave is the unit of scheduling: it owns
-//   one batch at a time, claims photon ids of that batch in chunks with one global atomic, and
-//   hands ids to its dead lanes (ballot + mbcnt), so lanes relaunch as soon as their photon ends.
-//   When its batch has no ids left, the wave drains its live photons, adds its LDS tally slot into
-//   the batch's HBM arrays and moves to the next batch that still has ids.
+// * Work comes in units: a range of photon ids of one layer stack, with the rows its scalars
+//   and profiles add into. A single run is one stack cut into its batches; a batched run is many
+//   stacks, each cut into batches. Ids are claimed within a unit in chunks, one global atomic
+//   per chunk.
+// * One workgroup per compute unit, W waves each. A wave owns one unit at a time and hands ids
+//   of the unit to its dead lanes (ballot + mbcnt), so lanes relaunch as soon as a photon ends.
+//   Waves take units from a global cursor first; once the cursor has passed the last unit, a wave
+//   looks for units that still have ids with a 64-lane parallel scan of the claim counters, so
+//   large units end up shared by many waves. When its unit has no ids left, the wave drains its
+//   live photons and adds its LDS slot into the unit's HBM rows.
 // * Tallies are integers in units of 2^-32 (a float32 weight >= 2^-8 converts exactly). Each
-//   wave has a private LDS slot for its current batch (scalars and A_z, scored at every event).
-//   R_r and T_r are scored once per photon and go straight to the batch's HBM arrays. The
-//   workgroup shares an LDS tile of A(r,z) as 32-bit counters: an add that wraps a counter
-//   carries 2^32 into the HBM bin (the LDS atomic returns the old value, so the carry is exact),
-//   which halves the tile's footprint so that it covers most of the grid; bins outside the tile
-//   go to HBM atomics. Integer sums do not depend on order, so a seed gives bit-identical
-//   tallies on every run.
+//   wave has a private LDS slot (scalars and A_z, scored at every event) and a private copy of
+//   its stack's layer table. R_r and T_r are scored once per photon, straight into HBM. For a
+//   single stack the workgroup also shares an LDS tile of A(r,z) as 32-bit counters: an add that
+//   wraps a counter carries 2^32 into the HBM bin (the LDS atomic returns the old value, so the
+//   carry is exact), which halves the tile's footprint so that it covers most of the grid; bins
+//   outside the tile go to HBM atomics. Integer sums do not depend on order, so a seed gives
+//   bit-identical tallies on every run and for every launch shape.
 // * Random words: each lane buffers two Philox blocks (8 words). At the top of every step it
 //   refills if fewer than 5 words are left, so the at-most-4 draws of a step never refill in the
 //   middle of divergent code; the words of the step are rotated into r0..r3 once.
@@ -33,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -41,7 +47,7 @@
 
 namespace tmc {
 
-// ------------------------------------------------------------------ per-layer record
+// ------------------------------------------------------------------ records
 
 enum LayerField {
     kMut = 0,    // mua + mus
@@ -66,31 +72,38 @@ static_assert(kLayerFields == 16, "layer record is 16 floats");
 
 constexpr int kCachedFields = 11;  // kMut..kHgE live in registers between layer changes
 
-// Slot layout (u64 words) of one wave's tally for its current batch.
+// Stack record (floats): this header, then kLayerFields floats per layer.
+enum StackHeader { kHdrW0 = 0, kHdrInvDz, kHdrInvDr, kHdrLayers, kStackHeader };
+
+// Slot layout (u64 words) of one wave's tally for its current unit: these scalars, then
+// absorbed[lmax], then A_z[nz].
 enum SlotScalar { kSlotRd = 0, kSlotTt, kSlotEvents, kSlotAnomalies, kSlotBoundary, kSlotAbs };
 
+// A unit of work: a range of photon ids of one stack and the rows its results add into.
+struct UnitRec {
+    uint64_t first;  // first photon id (ids index the stack's stream)
+    uint64_t count;
+    uint32_t stack;  // stack record
+    uint32_t prow;   // profile row (A_z, R_r, T_r)
+    uint32_t srow;   // scalar row (rd, tt, absorbed, counters)
+    uint32_t word3;  // Philox counter word 3 of the stack's streams
+};
+static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
+
 struct KernelArgs {
-    float layers[kMaxLayers * kLayerFields];
-    int32_t num_layers, nz, nr, batches;
-    float inv_dz, inv_dr, w0;
-    uint32_t key0, key1, word3;
-    uint64_t base, extra;  // batch plan: batch b holds base (+1 if b < extra) photons
-    uint64_t *next;        // [B] ids claimed so far in each batch
+    const UnitRec *units;
+    const float *stacks;
+    const uint64_t *key;  // [1] seed, in device memory so that a captured graph can be replayed
+    uint64_t *next;       // [U] ids claimed so far in each unit
+    uint64_t *cursor;     // [1] units handed out so far
     uint64_t *rd, *tt, *absorbed, *events, *anomalies, *bsteps, *az, *rr, *tr, *arz;
-    int32_t slot_words;  // u64 words per wave slot
-    int32_t off_az;
+    uint32_t n_units, stack_stride;
+    int32_t lmax, abs_stride, nz, nr;
+    int32_t slot_words, wave_words;    // u64 words: the slot, and slot + layer table, per wave
     int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] u32 at word tile_off
-    int32_t layer_off;                 // word offset of the layer table (floats) in LDS
     uint32_t chunk;
     uint32_t step_limit;
 };
-
-__device__ __forceinline__ uint64_t plan_first(const KernelArgs &a, uint64_t b) {
-    return b * a.base + (b < a.extra ? b : a.extra);
-}
-__device__ __forceinline__ uint64_t plan_count(const KernelArgs &a, uint64_t b) {
-    return a.base + (b < a.extra ? 1 : 0);
-}
 
 __device__ __forceinline__ uint64_t readfirstlane_u64(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
@@ -132,21 +145,99 @@ __device__ __forceinline__ uint32_t bad_score(float ux, float uy, float uz, floa
     return (!(w == w) || !(dev <= 1e-5f)) ? 1u : 0u;
 }
 
-// Add a wave's slot into the HBM arrays of batch b and clear it. All 64 lanes take part.
-__device__ void flush_slot(const KernelArgs &a, uint64_t *slot, uint64_t b, int lane) {
-    const int L = a.num_layers;
+// The wave's current unit (wave-uniform values).
+struct WaveUnit {
+    uint64_t cur, end;  // ids of the current chunk not handed out yet
+    uint32_t u, stack, prow, srow, word3;
+    float w0, inv_dz, inv_dr;
+    int nl;
+};
+
+// Claim the next chunk of unit u for this wave; false if u has no ids left.
+__device__ __forceinline__ bool claim_chunk(const KernelArgs &a, uint32_t u, int lane, WaveUnit &wu) {
+    uint64_t off = 0;
+    if (lane == 0)
+        off = __hip_atomic_fetch_add(a.next + u, static_cast<uint64_t>(a.chunk), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+    off = readfirstlane_u64(off);
+    const UnitRec rec = a.units[u];
+    if (off >= rec.count) return false;
+    wu.u = u;
+    wu.cur = rec.first + off;
+    wu.end = rec.first + std::min<uint64_t>(off + a.chunk, rec.count);
+    wu.stack = rec.stack;
+    wu.prow = rec.prow;
+    wu.srow = rec.srow;
+    wu.word3 = rec.word3;
+    return true;
+}
+
+// Find a unit with ids left: the cursor hands each unit out once; after that, a parallel scan of
+// the claim counters, 64 units per round, from a wave-specific offset.
+__device__ bool acquire_unit(const KernelArgs &a, int lane, uint32_t scan_start, bool &cursor_done,
+                             WaveUnit &wu) {
+    while (!cursor_done) {
+        uint64_t c = 0;
+        if (lane == 0)
+            c = __hip_atomic_fetch_add(a.cursor, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c = readfirstlane_u64(c);
+        if (c >= a.n_units) {
+            cursor_done = true;
+            break;
+        }
+        if (claim_chunk(a, static_cast<uint32_t>(c), lane, wu)) return true;
+    }
+    for (uint32_t k = 0; k < a.n_units; k += 64) {
+        const uint32_t i = k + static_cast<uint32_t>(lane);
+        uint32_t u = 0;
+        bool has = false;
+        if (i < a.n_units) {
+            u = (scan_start + i) % a.n_units;
+            const uint64_t claimed =
+                __hip_atomic_load(a.next + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            has = claimed < a.units[u].count;
+        }
+        uint64_t m = __ballot(has);
+        while (m != 0) {
+            const int l = __builtin_ctzll(m);
+            const uint32_t uc =
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u), l));
+            if (claim_chunk(a, uc, lane, wu)) return true;
+            m &= m - 1;
+        }
+    }
+    return false;
+}
+
+// Copy the unit's stack into the wave's LDS layer table, and its header into registers.
+__device__ __forceinline__ void load_stack(const KernelArgs &a, WaveUnit &wu, float *ltab, int lane) {
+    const float *rec = a.stacks + static_cast<uint64_t>(wu.stack) * a.stack_stride;
+    wu.w0 = rec[kHdrW0];
+    wu.inv_dz = rec[kHdrInvDz];
+    wu.inv_dr = rec[kHdrInvDr];
+    wu.nl = static_cast<int>(rec[kHdrLayers]);
+    for (int j = lane; j < wu.nl * kLayerFields; j += 64) ltab[j] = rec[kStackHeader + j];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+// Add a wave's slot into the HBM rows of its unit and clear it. All 64 lanes take part.
+__device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, const WaveUnit &wu,
+                                           int lane) {
+    const int L = a.lmax;
     for (int j = lane; j < a.slot_words; j += 64) {
         const uint64_t v = slot[j];
         if (v == 0) continue;
         slot[j] = 0;
         uint64_t *dst;
-        if (j == kSlotRd) dst = a.rd + b;
-        else if (j == kSlotTt) dst = a.tt + b;
-        else if (j == kSlotEvents) dst = a.events + b;
-        else if (j == kSlotAnomalies) dst = a.anomalies + b;
-        else if (j == kSlotBoundary) dst = a.bsteps + b;
-        else if (j < kSlotAbs + L) dst = a.absorbed + b * L + (j - kSlotAbs);
-        else dst = a.az + b * a.nz + (j - a.off_az);
+        if (j == kSlotRd) dst = a.rd + wu.srow;
+        else if (j == kSlotTt) dst = a.tt + wu.srow;
+        else if (j == kSlotEvents) dst = a.events + wu.srow;
+        else if (j == kSlotAnomalies) dst = a.anomalies + wu.srow;
+        else if (j == kSlotBoundary) dst = a.bsteps + wu.srow;
+        else if (j < kSlotAbs + L)
+            dst = a.absorbed + static_cast<uint64_t>(wu.srow) * a.abs_stride + (j - kSlotAbs);
+        else
+            dst = a.az + static_cast<uint64_t>(wu.prow) * a.nz + (j - (kSlotAbs + L));
         hbm_add(dst, v);
     }
 }
@@ -158,26 +249,28 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     const int wid = static_cast<int>(threadIdx.x >> 6);
     const int waves = static_cast<int>(blockDim.x >> 6);
 
-    // Clear the wave slots and the A(r,z) tile; copy the layer table into LDS.
-    const int words = waves * a.slot_words + (kArz ? (a.tile_z * a.tile_r + 1) / 2 : 0);
+    // Clear the wave regions and the A(r,z) tile.
+    const int words = waves * a.wave_words + (kArz ? (a.tile_z * a.tile_r + 1) / 2 : 0);
     for (int i = static_cast<int>(threadIdx.x); i < words; i += static_cast<int>(blockDim.x))
         lds[i] = 0;
-    float *ltab = reinterpret_cast<float *>(lds + a.layer_off);
-    for (int i = static_cast<int>(threadIdx.x); i < a.num_layers * kLayerFields;
-         i += static_cast<int>(blockDim.x))
-        ltab[i] = a.layers[i];
     __syncthreads();
 
-    uint64_t *slot = lds + wid * a.slot_words;
+    uint64_t *slot = lds + wid * a.wave_words;
+    float *ltab = reinterpret_cast<float *>(slot + a.slot_words);
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
-    const int last_layer = a.num_layers - 1;
+    const int off_az = kSlotAbs + a.lmax;
+    const uint64_t seed = a.key[0];
+    const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
 
     // ---- wave-uniform scheduling state
-    const uint64_t total_waves = static_cast<uint64_t>(gridDim.x) * waves;
-    const uint64_t gw = static_cast<uint64_t>(blockIdx.x) * waves + wid;
-    uint64_t b = (gw * static_cast<uint64_t>(a.batches)) / total_waves;
-    uint64_t first_b = plan_first(a, b), count_b = plan_count(a, b);
-    uint64_t cur = 0, end = 0;
+    const uint32_t total_waves = gridDim.x * static_cast<uint32_t>(waves);
+    const uint32_t gw = blockIdx.x * static_cast<uint32_t>(waves) + static_cast<uint32_t>(wid);
+    const uint32_t scan_start =
+        static_cast<uint32_t>((static_cast<uint64_t>(gw) * a.n_units) / total_waves);
+    bool cursor_done = false;
+    WaveUnit wu;
+    bool have = acquire_unit(a, lane, scan_start, cursor_done, wu);
+    if (have) load_stack(a, wu, ltab, lane);
     bool draining = false;
 
     // ---- lane state
@@ -192,39 +285,30 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
     uint64_t acc = 0;
 
-    for (;;) {
-        // ---- relaunch: hand the next ids of the wave's batch to lanes without a photon
+    while (have) {
+        // ---- relaunch: hand the next ids of the wave's unit to lanes without a photon
         uint64_t need = __ballot(!alive);
         while (need != 0 && !draining) {
-            if (cur >= end) {
-                uint64_t off = 0;
-                if (lane == 0)
-                    off = __hip_atomic_fetch_add(a.next + b, static_cast<uint64_t>(a.chunk),
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                off = readfirstlane_u64(off);
-                if (off >= count_b) {
-                    draining = true;
-                    break;
-                }
-                cur = first_b + off;
-                end = first_b + std::min<uint64_t>(off + a.chunk, count_b);
+            if (wu.cur >= wu.end && !claim_chunk(a, wu.u, lane, wu)) {
+                draining = true;
+                break;
             }
             const uint32_t n = static_cast<uint32_t>(__popcll(need));
-            const uint64_t avail = end - cur;
+            const uint64_t avail = wu.end - wu.cur;
             const uint32_t m = avail < n ? static_cast<uint32_t>(avail) : n;
             const uint32_t rank = lane_rank(need);
             if (!alive && rank < m) {
-                const uint64_t id = cur + rank;
+                const uint64_t id = wu.cur + rank;
                 id_lo = static_cast<uint32_t>(id);
                 id_hi = static_cast<uint32_t>(id >> 32);
-                const Block4 blk0 = philox4x32_10(id_lo, id_hi, 0u, a.word3, a.key0, a.key1);
+                const Block4 blk0 = philox4x32_10(id_lo, id_hi, 0u, wu.word3, key0, key1);
                 q4 = blk0.v0, q5 = blk0.v1, q6 = blk0.v2, q7 = blk0.v3;
                 blk = 1;
                 idx = 4;
                 x = y = z = 0.0f;
                 ux = uy = 0.0f;
                 uz = 1.0f;
-                w = a.w0;
+                w = wu.w0;
                 sleft = 0.0f;
                 layer = 0;
 #pragma unroll
@@ -232,44 +316,28 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 nstep = 0;
                 alive = true;
             }
-            cur += m;
+            wu.cur += m;
             need = __ballot(!alive);
         }
 
         if (draining && __ballot(alive) == 0) {
-            // The batch is done for this wave: publish its slot, then find a batch with ids left.
+            // The unit is done for this wave: publish its slot, then find another unit.
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            flush_slot(a, slot, b, lane);
+            flush_slot(a, slot, wu, lane);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            bool found = false;
-            for (int k = 1; k <= a.batches && !found; ++k) {
-                const uint64_t bb = (b + static_cast<uint64_t>(k)) % static_cast<uint64_t>(a.batches);
-                const uint64_t cnt = plan_count(a, bb);
-                uint64_t off = 0;
-                if (lane == 0)
-                    off = __hip_atomic_fetch_add(a.next + bb, static_cast<uint64_t>(a.chunk),
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                off = readfirstlane_u64(off);
-                if (off < cnt) {
-                    b = bb;
-                    first_b = plan_first(a, bb);
-                    count_b = cnt;
-                    cur = first_b + off;
-                    end = first_b + std::min<uint64_t>(off + a.chunk, cnt);
-                    found = true;
-                }
-            }
-            if (!found) break;
+            have = acquire_unit(a, lane, scan_start, cursor_done, wu);
+            if (have) load_stack(a, wu, ltab, lane);
             draining = false;
             continue;
         }
 
         if (!alive) continue;
 
+
         // ---- one hop-drop-spin step (physics.hpp:264-277)
         if (idx >= 4) {
             q0 = q4, q1 = q5, q2 = q6, q3 = q7;
-            const Block4 nb = philox4x32_10(id_lo, id_hi, blk, a.word3, a.key0, a.key1);
+            const Block4 nb = philox4x32_10(id_lo, id_hi, blk, wu.word3, key0, key1);
             q4 = nb.v0, q5 = nb.v1, q6 = nb.v2, q7 = nb.v3;
             ++blk;
             idx -= 4;
@@ -345,15 +413,15 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 uy *= ratio;
                 uz = up ? -ca2 : ca2;
                 const bool exit_top = up && layer == 0;
-                const bool exit_bot = !up && layer == last_layer;
+                const bool exit_bot = !up && layer == wu.nl - 1;
                 if (exit_top || exit_bot) {
                     an += bad_score(ux, uy, uz, w);
                     const uint64_t qw = to_fixed(w);
                     lds_add(slot + (exit_top ? kSlotRd : kSlotTt), qw);
                     if (kProfiles) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                        const int ir = bin_of(rr, a.inv_dr, a.nr);
-                        hbm_add((exit_top ? a.rr : a.tr) + b * a.nr + ir, qw);
+                        const int ir = bin_of(rr, wu.inv_dr, a.nr);
+                        hbm_add((exit_top ? a.rr : a.tr) + static_cast<uint64_t>(wu.prow) * a.nr + ir, qw);
                     }
                     died = true;
                 } else {
@@ -378,11 +446,11 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             const uint64_t q = to_fixed(dwa);
             acc += q;
             if (kProfiles) {
-                const int iz = bin_of(z, a.inv_dz, a.nz);
-                lds_add(slot + a.off_az + iz, q);
+                const int iz = bin_of(z, wu.inv_dz, a.nz);
+                lds_add(slot + off_az + iz, q);
                 if (kArz) {
                     const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                    const int ir = bin_of(rr, a.inv_dr, a.nr);
+                    const int ir = bin_of(rr, wu.inv_dr, a.nr);
                     uint64_t *hbm_bin = a.arz + static_cast<uint64_t>(iz) * a.nr + ir;
                     if (iz < a.tile_z && ir < a.tile_r) {
                         const uint32_t lo = static_cast<uint32_t>(q);
@@ -471,6 +539,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     }
 }
 
+// Clears n words. Used instead of a memset so that the clear can be captured in a graph.
+__global__ void clear_words(uint64_t *p, uint64_t n) {
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0;
+}
+
 __global__ void philox_stream_kernel(uint32_t key0, uint32_t key1, uint32_t id_lo, uint32_t id_hi,
                                      uint32_t word3, int64_t n, uint32_t *out) {
     const int64_t blocks = (n + 3) / 4;
@@ -485,40 +560,38 @@ __global__ void philox_stream_kernel(uint32_t key0, uint32_t key1, uint32_t id_l
 
 // ------------------------------------------------------------------ host side
 
-#define HIP_TRY(call)                                                                          \
-    do {                                                                                       \
-        const hipError_t e_ = (call);                                                          \
-        if (e_ != hipSuccess)                                                                  \
+#define HIP_TRY(call)                                                                              \
+    do {                                                                                           \
+        const hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                      \
             throw EngineError(TMC_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_) + \
                                                    " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
     } while (0)
 
 namespace {
 
-struct DeviceBuffer {
-    void *p = nullptr;
-    DeviceBuffer() = default;
-    explicit DeviceBuffer(size_t bytes) {
-        if (bytes) HIP_TRY(hipMalloc(&p, bytes));
-    }
-    ~DeviceBuffer() {
-        if (p) (void)hipFree(p);
-    }
-    DeviceBuffer(const DeviceBuffer &) = delete;
-    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
+double critical_cos(double ni, double nt) {
+    return ni > nt ? std::sqrt(1.0 - (nt * nt) / (ni * ni)) : 0.0;
+}
 
-double critical_cos(double ni, double nt) { return ni > nt ? std::sqrt(1.0 - (nt * nt) / (ni * ni)) : 0.0; }
-
-// The reference's make_geometry (config.hpp:363-390) and the per-layer constants the kernel keeps.
-void fill_layers(const tmc_spec &s, float *rec) {
+// One stack record: header and per-layer constants, derived like the reference's make_geometry
+// (config.hpp:363-390: depths accumulated in double, then cast).
+void fill_stack(int nl, double n_above, double n_below, const tmc_layer *layers, double dz,
+                double dr, float *rec) {
+    {
+        const float n0 = static_cast<float>(n_above), n1 = static_cast<float>(layers[0].n);
+        const float r = (n0 - n1) / (n0 + n1);
+        rec[kHdrW0] = 1.0f - r * r;  // launch weight in float, as physics.hpp:62 computes it
+    }
+    rec[kHdrInvDz] = 1.0f / static_cast<float>(dz);
+    rec[kHdrInvDr] = 1.0f / static_cast<float>(dr);
+    rec[kHdrLayers] = static_cast<float>(nl);
     double zacc = 0.0;
-    for (int i = 0; i < s.num_layers; ++i) {
-        const tmc_layer &L = s.layers[i];
-        const double n_top = i == 0 ? s.n_above : s.layers[i - 1].n;
-        const double n_bot = i + 1 == s.num_layers ? s.n_below : s.layers[i + 1].n;
-        float *r = rec + i * kLayerFields;
+    for (int i = 0; i < nl; ++i) {
+        const tmc_layer &L = layers[i];
+        const double n_top = i == 0 ? n_above : layers[i - 1].n;
+        const double n_bot = i + 1 == nl ? n_below : layers[i + 1].n;
+        float *r = rec + kStackHeader + i * kLayerFields;
         const float mua = static_cast<float>(L.mua), mus = static_cast<float>(L.mus);
         const float g = static_cast<float>(L.g);
         const float mut = mua + mus;
@@ -568,12 +641,251 @@ DeviceInfo device_info(int device) {
     return d;
 }
 
-template <bool P, bool R> void set_lds_limit(size_t bytes) {
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&transport_kernel<P, R>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+template <bool P, bool R> const void *kernel_ptr() {
+    return reinterpret_cast<const void *>(&transport_kernel<P, R>);
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------ resident engine state
+
+// Everything the kernel needs for one problem shape, on the host.
+struct Problem {
+    std::vector<UnitRec> units;
+    std::vector<float> stacks;  // n_stacks * stack_stride floats
+    uint32_t stack_stride = 0;
+    int lmax = 1, abs_stride = 1, nz = 1, nr = 1;
+    size_t n_srows = 0, n_prows = 0;
+    bool profiles = true, arz = false;
+    uint64_t photons = 0;
+    int reach_z = 1;  // depth bins that can receive weight (for the A(r,z) tile)
+};
+
+// Device memory, kernel arguments and launch shape for one problem shape. A batched plan keeps
+// one of these and replays a captured graph (clear + transport) on it.
+class Resident {
+  public:
+    Resident(const DeviceInfo &dev, const Problem &p, const tmc_tuning &t) : dev_(dev) {
+        std::memset(&a_, 0, sizeof a_);
+        a_.n_units = static_cast<uint32_t>(p.units.size());
+        a_.stack_stride = p.stack_stride;
+        a_.lmax = p.lmax;
+        a_.abs_stride = p.abs_stride;
+        a_.nz = p.nz;
+        a_.nr = p.nr;
+        a_.step_limit = t.step_limit ? static_cast<uint32_t>(std::min<uint64_t>(t.step_limit, 0xffffffffu))
+                                     : 1000000000u;
+        profiles_ = p.profiles;
+        arz_ = p.arz;
+        a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
+        a_.wave_words = a_.slot_words + (p.lmax * kLayerFields + 1) / 2;
+
+        const size_t cap = dev.lds_per_block;
+        int wpg = t.waves_per_group > 0 ? t.waves_per_group : 16;
+        if (wpg > 16) throw EngineError(TMC_ERR_CONFIG, "waves_per_group must be <= 16");
+        auto wave_bytes = [&](int w) { return static_cast<size_t>(w) * a_.wave_words * 8; };
+        while (wpg > 1 && wave_bytes(wpg) > cap) --wpg;
+        if (wave_bytes(wpg) > cap)
+            throw EngineError(TMC_ERR_CONFIG,
+                              "the profile grid is too large for one wave's LDS tally (" +
+                                  std::to_string(wave_bytes(1)) + " bytes; the device has " +
+                                  std::to_string(cap) + " per workgroup)");
+        int tile_z = 0, tile_r = 0;
+        if (p.arz) {
+            const size_t free_bins = (cap - wave_bytes(wpg)) / 4;  // u32 counters
+            if (t.arz_tile_z >= 0 && t.arz_tile_r >= 0) {
+                tile_z = std::min(t.arz_tile_z, p.nz);
+                tile_r = std::min(t.arz_tile_r, p.nr);
+                if (static_cast<size_t>(tile_z) * tile_r > free_bins)
+                    throw EngineError(TMC_ERR_CONFIG, "A(r,z) tile does not fit in LDS");
+            } else {
+                // Cover every reachable depth and as many radii from the axis as fit (deposits
+                // thin out with r); fall back to a square-ish tile when that is too narrow.
+                tile_z = p.reach_z;
+                tile_r = static_cast<int>(std::min<size_t>(p.nr, free_bins / std::max(tile_z, 1)));
+                if (tile_r < std::min(p.nr, 32)) {
+                    tile_r = static_cast<int>(std::min<size_t>(
+                        p.nr, std::max<size_t>(1, static_cast<size_t>(std::sqrt(static_cast<double>(free_bins))))));
+                    tile_z = static_cast<int>(std::min<size_t>(p.reach_z, free_bins / std::max(tile_r, 1)));
+                }
+            }
+        }
+        a_.tile_z = tile_z;
+        a_.tile_r = tile_r;
+        a_.tile_off = wpg * a_.wave_words;
+        lds_bytes_ = (static_cast<size_t>(a_.tile_off) + (static_cast<size_t>(tile_z) * tile_r + 1) / 2) * 8;
+        wpg_ = wpg;
+        groups_ = t.groups > 0 ? t.groups : dev.cus;
+        const uint64_t waves_total = static_cast<uint64_t>(groups_) * wpg;
+        a_.chunk = t.chunk > 0 ? static_cast<uint32_t>(t.chunk)
+                               : static_cast<uint32_t>(std::clamp<uint64_t>(p.photons / (waves_total * 8), 64, 4096));
+
+        // Device memory: inputs, then everything the clear kernel zeroes in one range.
+        const size_t U = p.units.size(), S = p.n_srows, PR = p.n_prows;
+        n_units_ = U;
+        n_stacks_floats_ = p.stacks.size();
+        units_ = alloc(U * sizeof(UnitRec));
+        stacks_ = alloc(p.stacks.size() * sizeof(float));
+        key_ = alloc(8);
+        out_words_ = 1 /*cursor*/ + U /*next*/ + 5 * S + S * p.abs_stride +
+                     (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
+                     (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0);
+        out_ = alloc(out_words_ * 8);
+        uint64_t *q = static_cast<uint64_t *>(out_);
+        a_.cursor = q; q += 1;
+        a_.next = q; q += U;
+        a_.rd = q; q += S;
+        a_.tt = q; q += S;
+        a_.events = q; q += S;
+        a_.anomalies = q; q += S;
+        a_.bsteps = q; q += S;
+        a_.absorbed = q; q += S * p.abs_stride;
+        if (p.profiles) {
+            a_.az = q; q += PR * p.nz;
+            a_.rr = q; q += PR * p.nr;
+            a_.tr = q; q += PR * p.nr;
+        }
+        if (p.arz) a_.arz = q;
+        a_.units = static_cast<const UnitRec *>(units_);
+        a_.stacks = static_cast<const float *>(stacks_);
+        a_.key = static_cast<const uint64_t *>(key_);
+        n_srows_ = S;
+        n_prows_ = PR;
+
+        HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&e0_));
+        HIP_TRY(hipEventCreate(&e1_));
+        const void *k = kernel();
+        HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(lds_bytes_)));
+    }
+
+    ~Resident() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (e0_) (void)hipEventDestroy(e0_);
+        if (e1_) (void)hipEventDestroy(e1_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+        for (void *p : allocs_) (void)hipFree(p);
+    }
+    Resident(const Resident &) = delete;
+    Resident &operator=(const Resident &) = delete;
+
+    void upload(const Problem &p, uint64_t seed) {
+        if (p.units.size() != n_units_ || p.stacks.size() != n_stacks_floats_)
+            throw EngineError(TMC_ERR_CONFIG, "the plan was made for another number of stacks");
+        seed_ = seed;
+        HIP_TRY(hipMemcpyAsync(units_, p.units.data(), p.units.size() * sizeof(UnitRec),
+                               hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipMemcpyAsync(stacks_, p.stacks.data(), p.stacks.size() * sizeof(float),
+                               hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipMemcpyAsync(key_, &seed_, 8, hipMemcpyHostToDevice, stream_));
+    }
+
+    // Clear the outputs and run the transport kernel; returns the kernel time in seconds (HIP
+    // events on this stream). With use_graph the two launches are captured once and replayed.
+    double run(bool use_graph) {
+        if (use_graph) {
+            if (!graph_exec_) {
+                HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                enqueue_clear();
+                enqueue_transport();
+                HIP_TRY(hipStreamEndCapture(stream_, &graph_));
+                HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+            }
+            HIP_TRY(hipEventRecord(e0_, stream_));
+            HIP_TRY(hipGraphLaunch(graph_exec_, stream_));
+            HIP_TRY(hipEventRecord(e1_, stream_));
+        } else {
+            enqueue_clear();
+            HIP_TRY(hipEventRecord(e0_, stream_));
+            enqueue_transport();
+            HIP_TRY(hipEventRecord(e1_, stream_));
+        }
+        HIP_TRY(hipEventSynchronize(e1_));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0_, e1_));
+        return static_cast<double>(ms) / 1000.0;
+    }
+
+    // Copy results to caller buffers (any may be null).
+    void download(uint64_t *rd, uint64_t *tt, uint64_t *absorbed, uint64_t *events,
+                  uint64_t *anomalies, uint64_t *bsteps, uint64_t *az, uint64_t *rr, uint64_t *tr,
+                  uint64_t *arz) {
+        auto d2h = [&](uint64_t *dst, const uint64_t *src, size_t n) {
+            if (dst && src && n) HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream_));
+        };
+        const size_t S = n_srows_, PR = n_prows_;
+        d2h(rd, a_.rd, S);
+        d2h(tt, a_.tt, S);
+        d2h(absorbed, a_.absorbed, S * a_.abs_stride);
+        d2h(events, a_.events, S);
+        d2h(anomalies, a_.anomalies, S);
+        d2h(bsteps, a_.bsteps, S);
+        if (profiles_) {
+            d2h(az, a_.az, PR * a_.nz);
+            d2h(rr, a_.rr, PR * a_.nr);
+            d2h(tr, a_.tr, PR * a_.nr);
+        } else {
+            if (az) std::memset(az, 0, PR * a_.nz * 8);
+            if (rr) std::memset(rr, 0, PR * a_.nr * 8);
+            if (tr) std::memset(tr, 0, PR * a_.nr * 8);
+        }
+        if (arz_) d2h(arz, a_.arz, static_cast<size_t>(a_.nz) * a_.nr);
+        HIP_TRY(hipStreamSynchronize(stream_));
+    }
+
+    int groups() const { return groups_; }
+    int waves_per_group() const { return wpg_; }
+    size_t lds_bytes() const { return lds_bytes_; }
+    uint32_t chunk() const { return a_.chunk; }
+    int tile_z() const { return a_.tile_z; }
+    int tile_r() const { return a_.tile_r; }
+    const DeviceInfo &device() const { return dev_; }
+
+  private:
+    void *alloc(size_t bytes) {
+        void *p = nullptr;
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+        allocs_.push_back(p);
+        return p;
+    }
+    const void *kernel() const {
+        if (!profiles_) return kernel_ptr<false, false>();
+        return arz_ ? kernel_ptr<true, true>() : kernel_ptr<true, false>();
+    }
+    void enqueue_clear() {
+        const unsigned grid = static_cast<unsigned>(std::min<size_t>((out_words_ + 255) / 256, 4096));
+        hipLaunchKernelGGL(clear_words, dim3(grid), dim3(256), 0, stream_,
+                           static_cast<uint64_t *>(out_), static_cast<uint64_t>(out_words_));
+        HIP_TRY(hipGetLastError());
+    }
+    void enqueue_transport() {
+        const dim3 grid(static_cast<unsigned>(groups_)), block(static_cast<unsigned>(wpg_ * 64));
+        if (!profiles_)
+            hipLaunchKernelGGL((transport_kernel<false, false>), grid, block, lds_bytes_, stream_, a_);
+        else if (arz_)
+            hipLaunchKernelGGL((transport_kernel<true, true>), grid, block, lds_bytes_, stream_, a_);
+        else
+            hipLaunchKernelGGL((transport_kernel<true, false>), grid, block, lds_bytes_, stream_, a_);
+        HIP_TRY(hipGetLastError());
+    }
+
+    DeviceInfo dev_;
+    KernelArgs a_;
+    bool profiles_ = true, arz_ = false;
+    int wpg_ = 16, groups_ = 0;
+    size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
+    void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
+    uint64_t seed_ = 0;
+    std::vector<void *> allocs_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+};
+
+// ------------------------------------------------------------------ single run
 
 void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -589,190 +901,210 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) is a profile: it cannot be combined with no-profiles");
     if (want_arz && out->arz == nullptr)
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
-
     const DeviceInfo dev = device_info(opt.device);
-    const int L = spec.num_layers, nz = spec.nz, nr = spec.nr;
+
+    Problem p;
+    const int L = spec.num_layers;
     const uint64_t B = static_cast<uint64_t>(opt.batches);
-
-    // ---- launch shape and LDS layout
-    KernelArgs a;
-    std::memset(&a, 0, sizeof a);
-    fill_layers(spec, a.layers);
-    a.num_layers = L;
-    a.nz = nz;
-    a.nr = nr;
-    a.batches = opt.batches;
-    a.inv_dz = 1.0f / static_cast<float>(spec.dz);
-    a.inv_dr = 1.0f / static_cast<float>(spec.dr);
-    {
-        const float n0 = static_cast<float>(spec.n_above), n1 = static_cast<float>(spec.layers[0].n);
-        const float r = (n0 - n1) / (n0 + n1);
-        a.w0 = 1.0f - r * r;  // launch weight in float, as physics.hpp:62 computes it
+    p.lmax = L;
+    p.abs_stride = L;
+    p.nz = spec.nz;
+    p.nr = spec.nr;
+    p.profiles = profiles;
+    p.arz = want_arz;
+    p.photons = spec.photons;
+    p.n_srows = B;
+    p.n_prows = B;
+    p.stack_stride = kStackHeader + L * kLayerFields;
+    p.stacks.assign(p.stack_stride, 0.0f);
+    fill_stack(L, spec.n_above, spec.n_below, spec.layers, spec.dz, spec.dr, p.stacks.data());
+    double depth = 0.0;
+    for (int i = 0; i < L; ++i) depth += spec.layers[i].d;
+    const double reach = std::floor(depth / spec.dz) + 2.0;
+    p.reach_z = reach < static_cast<double>(spec.nz) ? static_cast<int>(reach) : spec.nz;
+    const uint64_t base = spec.photons / B, extra = spec.photons % B;
+    for (uint64_t b = 0; b < B; ++b) {
+        UnitRec u;
+        u.first = b * base + (b < extra ? b : extra);  // the reference's BatchPlan::first_of
+        u.count = base + (b < extra ? 1 : 0);
+        u.stack = 0;
+        u.prow = static_cast<uint32_t>(b);
+        u.srow = static_cast<uint32_t>(b);
+        u.word3 = opt.stream;
+        p.units.push_back(u);
     }
-    a.key0 = static_cast<uint32_t>(opt.seed);
-    a.key1 = static_cast<uint32_t>(opt.seed >> 32);
-    a.word3 = opt.stream;
-    a.base = spec.photons / B;
-    a.extra = spec.photons % B;
-    a.step_limit = opt.step_limit ? static_cast<uint32_t>(std::min<uint64_t>(opt.step_limit, 0xffffffffu))
-                                  : 1000000000u;
+    tmc_tuning t;
+    t.waves_per_group = opt.waves_per_group;
+    t.groups = opt.groups;
+    t.chunk = opt.chunk;
+    t.arz_tile_z = opt.arz_tile_z;
+    t.arz_tile_r = opt.arz_tile_r;
+    t.step_limit = opt.step_limit;
 
-    a.off_az = kSlotAbs + L;
-    a.slot_words = a.off_az + (profiles ? nz : 0);
-
-    const size_t lds_cap = dev.lds_per_block;
-    const size_t layer_bytes = static_cast<size_t>(L) * kLayerFields * sizeof(float);
-    int wpg = opt.waves_per_group > 0 ? opt.waves_per_group : 16;
-    if (wpg > 16) throw EngineError(TMC_ERR_CONFIG, "waves_per_group must be <= 16");
-    auto slots_bytes = [&](int w) { return static_cast<size_t>(w) * a.slot_words * 8; };
-    while (wpg > 1 && slots_bytes(wpg) + layer_bytes > lds_cap) --wpg;
-    if (slots_bytes(wpg) + layer_bytes > lds_cap)
-        throw EngineError(TMC_ERR_CONFIG,
-                          "the profile grid is too large for one wave's LDS tally (" +
-                              std::to_string(slots_bytes(1)) + " bytes; the device has " +
-                              std::to_string(lds_cap) + " per workgroup)");
-    int tile_z = 0, tile_r = 0;
-    if (want_arz) {
-        const size_t free_bins = (lds_cap - slots_bytes(wpg) - layer_bytes) / 4;  // u32 counters
-        if (opt.arz_tile_z >= 0 && opt.arz_tile_r >= 0) {
-            tile_z = std::min(opt.arz_tile_z, nz);
-            tile_r = std::min(opt.arz_tile_r, nr);
-            if (static_cast<size_t>(tile_z) * tile_r > free_bins)
-                throw EngineError(TMC_ERR_CONFIG, "A(r,z) tile does not fit in LDS");
-        } else {
-            // Depth bins below the stack never receive weight; cover every reachable depth and
-            // as many radii from the axis outward as fit (deposits thin out with r).
-            double depth = 0.0;
-            for (int i = 0; i < L; ++i) depth += spec.layers[i].d;
-            const double reach = std::floor(depth / spec.dz) + 2.0;
-            const int nz_eff = reach < static_cast<double>(nz) ? static_cast<int>(reach) : nz;
-            tile_z = nz_eff;
-            tile_r = static_cast<int>(std::min<size_t>(nr, free_bins / std::max(tile_z, 1)));
-            if (tile_r < std::min(nr, 32)) {
-                tile_r = static_cast<int>(std::min<size_t>(nr, std::max<size_t>(1, static_cast<size_t>(std::sqrt(static_cast<double>(free_bins))))));
-                tile_z = static_cast<int>(std::min<size_t>(nz_eff, free_bins / std::max(tile_r, 1)));
-            }
-        }
-    }
-    a.tile_z = tile_z;
-    a.tile_r = tile_r;
-    a.tile_off = wpg * a.slot_words;
-    a.layer_off = a.tile_off + (tile_z * tile_r + 1) / 2;
-    const size_t lds_bytes = static_cast<size_t>(a.layer_off) * 8 + layer_bytes;
-
-    const int groups = opt.groups > 0 ? opt.groups : dev.cus;
-    const uint64_t lanes = static_cast<uint64_t>(groups) * wpg * 64;
-    uint32_t chunk = opt.chunk > 0 ? static_cast<uint32_t>(opt.chunk) : 0;
-    if (chunk == 0) {
-        // Enough claims per wave to balance the tail, few enough to keep the atomic rare.
-        const uint64_t per_wave = spec.photons / std::max<uint64_t>(lanes / 64, 1);
-        chunk = static_cast<uint32_t>(std::clamp<uint64_t>(per_wave / 8, 64, 4096));
-    }
-    a.chunk = chunk;
-
-    // ---- device buffers
-    HIP_TRY(hipSetDevice(dev.id));
-    hipStream_t stream;
-    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg{stream};
-    const size_t nB = B, nBL = B * L, nBz = B * nz, nBr = B * nr;
-    const size_t n_arz = want_arz ? static_cast<size_t>(nz) * nr : 0;
-    const size_t total_words = B /*next*/ + 6 * nB + nBL + nBz + 2 * nBr + n_arz;
-    DeviceBuffer buf(total_words * 8);
-    uint64_t *p = buf.as<uint64_t>();
-    a.next = p; p += nB;
-    a.rd = p; p += nB;
-    a.tt = p; p += nB;
-    a.events = p; p += nB;
-    a.anomalies = p; p += nB;
-    a.bsteps = p; p += nB;
-    p += nB;  // spare, keeps the layout 16-byte aligned for every array
-    a.absorbed = p; p += nBL;
-    a.az = p; p += nBz;
-    a.rr = p; p += nBr;
-    a.tr = p; p += nBr;
-    a.arz = want_arz ? p : nullptr;
-    HIP_TRY(hipMemsetAsync(buf.p, 0, total_words * 8, stream));
-
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard() {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        }
-    } eg{e0, e1};
-
-    const dim3 grid(static_cast<unsigned>(groups)), block(static_cast<unsigned>(wpg * 64));
-    HIP_TRY(hipEventRecord(e0, stream));
-    if (!profiles) {
-        set_lds_limit<false, false>(lds_bytes);
-        hipLaunchKernelGGL((transport_kernel<false, false>), grid, block, lds_bytes, stream, a);
-    } else if (want_arz) {
-        set_lds_limit<true, true>(lds_bytes);
-        hipLaunchKernelGGL((transport_kernel<true, true>), grid, block, lds_bytes, stream, a);
-    } else {
-        set_lds_limit<true, false>(lds_bytes);
-        hipLaunchKernelGGL((transport_kernel<true, false>), grid, block, lds_bytes, stream, a);
-    }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(e1, stream));
-    HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-
-    auto d2h = [&](uint64_t *dst, const uint64_t *src, size_t n) {
-        if (dst && n) HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream));
-    };
-    d2h(out->rd, a.rd, nB);
-    d2h(out->tt, a.tt, nB);
-    d2h(out->events, a.events, nB);
-    d2h(out->anomalies, a.anomalies, nB);
-    d2h(out->boundary_steps, a.bsteps, nB);
-    d2h(out->absorbed, a.absorbed, nBL);
-    d2h(out->az, a.az, nBz);
-    d2h(out->rr, a.rr, nBr);
-    d2h(out->tr, a.tr, nBr);
-    if (want_arz) d2h(out->arz, a.arz, n_arz);
-    HIP_TRY(hipStreamSynchronize(stream));
-
+    Resident r(dev, p, t);
+    r.upload(p, opt.seed);
+    out->kernel_seconds = r.run(false);
+    r.download(out->rd, out->tt, out->absorbed, out->events, out->anomalies, out->boundary_steps,
+               out->az, out->rr, out->tr, want_arz ? out->arz : nullptr);
     {
         const double n0 = spec.n_above, n1 = spec.layers[0].n;
-        const double r = (n0 - n1) / (n0 + n1);
-        out->specular = r * r;  // host double, as the reference reports it
+        const double rr = (n0 - n1) / (n0 + n1);
+        out->specular = rr * rr;  // host double, as the reference reports it
     }
-    out->kernel_seconds = static_cast<double>(ms) / 1000.0;
     std::snprintf(out->device, sizeof out->device, "%s", dev.name.c_str());
-    out->groups = groups;
-    out->waves_per_group = wpg;
-    out->lds_bytes = static_cast<int32_t>(lds_bytes);
-    out->chunk = static_cast<int32_t>(chunk);
-    out->arz_tile_z = tile_z;
-    out->arz_tile_r = tile_r;
-    out->wall_seconds =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    out->groups = r.groups();
+    out->waves_per_group = r.waves_per_group();
+    out->lds_bytes = static_cast<int32_t>(r.lds_bytes());
+    out->chunk = static_cast<int32_t>(r.chunk());
+    out->arz_tile_z = r.tile_z();
+    out->arz_tile_r = r.tile_r();
+    out->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// ------------------------------------------------------------------ batched runs
+
+struct BatchedPlan {
+    DeviceInfo dev;
+    Problem shape;
+    tmc_batched_opts opt;
+    std::unique_ptr<Resident> res;
+};
+
+namespace {
+
+void check_stack(const tmc_stack &s, int i) {
+    tmc_spec spec;
+    std::memset(&spec, 0, sizeof spec);
+    spec.photons = s.photons;
+    spec.dz = spec.dr = 1.0;
+    spec.nz = spec.nr = spec.na = 1;
+    spec.n_above = s.n_above;
+    spec.n_below = s.n_below;
+    spec.num_layers = s.num_layers;
+    std::memcpy(spec.layers, s.layers, sizeof spec.layers);
+    try {
+        validate_spec(spec);
+    } catch (const EngineError &e) {
+        throw EngineError(TMC_ERR_CONFIG, "stack " + std::to_string(i) + ": " + e.what());
+    }
+}
+
+// Units of a batched run: stack s, batch b covers the ids of the reference's BatchPlan over the
+// stack's photon budget; the stream of stack s uses counter word 3 = s. Units are listed from
+// the largest budget down, so the longest work starts first.
+void batched_units(const tmc_stack *stacks, int n, int batches, Problem &p) {
+    p.units.clear();
+    for (int s = 0; s < n; ++s) {
+        const uint64_t N = stacks[s].photons, B = static_cast<uint64_t>(batches);
+        const uint64_t base = N / B, extra = N % B;
+        for (uint64_t b = 0; b < B; ++b) {
+            UnitRec u;
+            u.first = b * base + (b < extra ? b : extra);
+            u.count = base + (b < extra ? 1 : 0);
+            u.stack = static_cast<uint32_t>(s);
+            u.prow = static_cast<uint32_t>(s);
+            u.srow = static_cast<uint32_t>(s * B + b);
+            u.word3 = static_cast<uint32_t>(s);
+            p.units.push_back(u);
+        }
+    }
+    std::stable_sort(p.units.begin(), p.units.end(),
+                     [](const UnitRec &x, const UnitRec &y) { return x.count > y.count; });
+    p.photons = 0;
+    for (int s = 0; s < n; ++s) p.photons += stacks[s].photons;
+}
+
+void batched_stacks(const tmc_stack *stacks, int n, const tmc_batched_opts &o, Problem &p) {
+    p.stacks.assign(static_cast<size_t>(n) * p.stack_stride, 0.0f);
+    for (int s = 0; s < n; ++s)
+        fill_stack(stacks[s].num_layers, stacks[s].n_above, stacks[s].n_below, stacks[s].layers,
+                   o.dz, o.dr, p.stacks.data() + static_cast<size_t>(s) * p.stack_stride);
+}
+
+}  // namespace
+
+BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &o) {
+    if (n_stacks < 1) throw EngineError(TMC_ERR_CONFIG, "n_stacks must be >= 1");
+    if (o.batches_per_stack < 1) throw EngineError(TMC_ERR_CONFIG, "batches_per_stack must be >= 1");
+    if (!(o.dz > 0) || !(o.dr > 0) || o.nz < 1 || o.nr < 1)
+        throw EngineError(TMC_ERR_CONFIG, "dz, dr must be > 0 and nz, nr >= 1");
+    if (static_cast<uint64_t>(n_stacks) * o.batches_per_stack > 0xffffffffull)
+        throw EngineError(TMC_ERR_CONFIG, "too many stacks x batches");
+    auto plan = std::make_unique<BatchedPlan>();
+    plan->dev = device_info(o.device);
+    plan->opt = o;
+    Problem &p = plan->shape;
+    p.lmax = kMaxLayers;
+    p.abs_stride = kMaxLayers;
+    p.nz = o.nz;
+    p.nr = o.nr;
+    p.profiles = (o.flags & TMC_FLAG_NO_PROFILES) == 0;
+    p.arz = false;
+    p.n_srows = static_cast<size_t>(n_stacks) * o.batches_per_stack;
+    p.n_prows = static_cast<size_t>(n_stacks);
+    p.stack_stride = kStackHeader + kMaxLayers * kLayerFields;
+    p.units.resize(static_cast<size_t>(n_stacks) * o.batches_per_stack);
+    p.stacks.resize(static_cast<size_t>(n_stacks) * p.stack_stride);
+    p.photons = o.photons_hint ? o.photons_hint : static_cast<uint64_t>(n_stacks) * 100000ull;
+    tmc_tuning t;
+    t.waves_per_group = o.waves_per_group;
+    t.groups = o.groups;
+    t.chunk = o.chunk;
+    t.arz_tile_z = t.arz_tile_r = 0;
+    t.step_limit = o.step_limit;
+    plan->res = std::make_unique<Resident>(plan->dev, p, t);
+    return plan.release();
+}
+
+void batched_execute(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, uint64_t seed,
+                     tmc_batched_out *out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    Problem &p = plan.shape;
+    if (static_cast<size_t>(n_stacks) != p.n_prows)
+        throw EngineError(TMC_ERR_CONFIG, "the plan was made for " + std::to_string(p.n_prows) +
+                                              " stacks, got " + std::to_string(n_stacks));
+    for (int s = 0; s < n_stacks; ++s) {
+        check_stack(stacks[s], s);
+        if (stacks[s].photons < static_cast<uint64_t>(plan.opt.batches_per_stack))
+            throw EngineError(TMC_ERR_CONFIG, "stack " + std::to_string(s) +
+                                                  ": batches cannot exceed the number of photons");
+    }
+    batched_units(stacks, n_stacks, plan.opt.batches_per_stack, p);
+    batched_stacks(stacks, n_stacks, plan.opt, p);
+    plan.res->upload(p, seed);
+    out->kernel_seconds = plan.res->run((plan.opt.flags & TMC_FLAG_NO_GRAPH) == 0);
+    plan.res->download(out->rd, out->tt, out->absorbed, out->events, out->anomalies,
+                       out->boundary_steps, out->az, out->rr, out->tr, nullptr);
+    std::snprintf(out->device, sizeof out->device, "%s", plan.dev.name.c_str());
+    out->groups = plan.res->groups();
+    out->waves_per_group = plan.res->waves_per_group();
+    out->lds_bytes = static_cast<int32_t>(plan.res->lds_bytes());
+    out->chunk = static_cast<int32_t>(plan.res->chunk());
+    out->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void batched_destroy(BatchedPlan *plan) { delete plan; }
+
+// ------------------------------------------------------------------ parity hook
 
 void philox_stream(uint64_t seed, uint64_t id, uint32_t word3, int64_t n, uint32_t *out,
                    int device) {
     if (n < 0) throw EngineError(TMC_ERR_CONFIG, "n must be >= 0");
     if (n == 0) return;
-    const DeviceInfo dev = device_info(device);
-    (void)dev;
-    DeviceBuffer d(static_cast<size_t>(n) * 4);
+    device_info(device);
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, static_cast<size_t>(n) * 4));
     const int64_t blocks = (n + 3) / 4;
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((blocks + 255) / 256, 1024));
     hipLaunchKernelGGL(philox_stream_kernel, dim3(grid), dim3(256), 0, 0,
                        static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32),
                        static_cast<uint32_t>(id), static_cast<uint32_t>(id >> 32), word3, n,
-                       d.as<uint32_t>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpy(out, d.p, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost));
+                       static_cast<uint32_t *>(d));
+    const hipError_t e = hipGetLastError();
+    const hipError_t c = hipMemcpy(out, d, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIP_TRY(e);
+    HIP_TRY(c);
 }
 
 bool device_available() {

@@ -18,7 +18,7 @@ MAX_LAYERS = 16
 UNIT = 2.0 ** -32  # weight of one tally unit
 
 TMC_OK, TMC_ERR_CONFIG, TMC_ERR_DEVICE, TMC_ERR_RUNTIME = 0, 1, 2, 3
-FLAG_ARZ, FLAG_NO_PROFILES = 1, 2
+FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH = 1, 2, 4
 
 
 class ConfigError(ValueError):
@@ -61,9 +61,40 @@ class TmcOut(C.Structure):
                 ("arz_tile_r", C.c_int32)]
 
 
+class TmcStack(C.Structure):
+    _fields_ = [("num_layers", C.c_int32), ("n_above", C.c_double), ("n_below", C.c_double),
+                ("layers", TmcLayer * MAX_LAYERS), ("photons", C.c_uint64)]
+
+
+class TmcBatchedOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("batches_per_stack", C.c_int32), ("device", C.c_int32),
+                ("dz", C.c_double), ("dr", C.c_double), ("nz", C.c_int32), ("nr", C.c_int32),
+                ("flags", C.c_uint32), ("waves_per_group", C.c_int32), ("groups", C.c_int32),
+                ("chunk", C.c_int32), ("step_limit", C.c_uint64), ("photons_hint", C.c_uint64)]
+
+
+class TmcBatchedOut(C.Structure):
+    _fields_ = [("rd", _PU64), ("tt", _PU64), ("absorbed", _PU64), ("events", _PU64),
+                ("anomalies", _PU64), ("boundary_steps", _PU64), ("az", _PU64), ("rr", _PU64),
+                ("tr", _PU64), ("kernel_seconds", C.c_double), ("wall_seconds", C.c_double),
+                ("device", C.c_char * 96), ("groups", C.c_int32), ("waves_per_group", C.c_int32),
+                ("lds_bytes", C.c_int32), ("chunk", C.c_int32)]
+
+
+class TmcStackDist(C.Structure):
+    _fields_ = [("min_layers", C.c_int32), ("max_layers", C.c_int32),
+                ("mua_lo", C.c_double), ("mua_hi", C.c_double), ("mus_lo", C.c_double),
+                ("mus_hi", C.c_double), ("g_lo", C.c_double), ("g_hi", C.c_double),
+                ("n_lo", C.c_double), ("n_hi", C.c_double), ("d_lo", C.c_double),
+                ("d_hi", C.c_double), ("photons_lo", C.c_double), ("photons_hi", C.c_double),
+                ("n_ambient", C.c_double)]
+
+
 EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_point_unit",
             "tmc_default_opts", "tmc_validate_spec", "tmc_parse_mci_file", "tmc_parse_mci_text",
-            "tmc_hip_philox_stream")
+            "tmc_hip_philox_stream", "tmc_batched_create", "tmc_batched_execute",
+            "tmc_batched_destroy", "tmc_hip_run_batched", "tmc_default_stack_dist",
+            "tmc_generate_stacks")
 
 _lib: C.CDLL | None = None
 
@@ -94,6 +125,20 @@ def load() -> C.CDLL:
     L.tmc_hip_philox_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64,
                                         C.POINTER(C.c_uint32), C.c_int32]
     L.tmc_hip_philox_stream.restype = C.c_int
+    L.tmc_batched_create.argtypes = [C.c_int32, C.POINTER(TmcBatchedOpts), C.POINTER(C.c_void_p)]
+    L.tmc_batched_create.restype = C.c_int
+    L.tmc_batched_execute.argtypes = [C.c_void_p, C.POINTER(TmcStack), C.c_int32, C.c_uint64,
+                                      C.POINTER(TmcBatchedOut)]
+    L.tmc_batched_execute.restype = C.c_int
+    L.tmc_batched_destroy.argtypes = [C.c_void_p]
+    L.tmc_batched_destroy.restype = None
+    L.tmc_hip_run_batched.argtypes = [C.POINTER(TmcStack), C.c_int32, C.POINTER(TmcBatchedOpts),
+                                      C.POINTER(TmcBatchedOut)]
+    L.tmc_hip_run_batched.restype = C.c_int
+    L.tmc_default_stack_dist.argtypes = [C.POINTER(TmcStackDist)]
+    L.tmc_generate_stacks.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(TmcStackDist),
+                                      C.POINTER(TmcStack)]
+    L.tmc_generate_stacks.restype = C.c_int
     _lib = L
     return L
 
@@ -276,3 +321,124 @@ def philox_stream(seed: int, photon_id: int, n: int, word3: int = 0, device: int
     if code != TMC_OK:
         _raise(code)
     return out
+
+
+# ------------------------------------------------------------------ batched runs and stacks
+
+def default_stack_dist(**overrides) -> TmcStackDist:
+    """The batched workload's stack distribution, with optional field overrides."""
+    d = TmcStackDist()
+    load().tmc_default_stack_dist(C.byref(d))
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
+def generate_stacks(seed: int, first_id: int, count: int, dist: TmcStackDist | None = None):
+    """Stacks first_id .. first_id + count - 1 of the sequence keyed by seed (a ctypes array)."""
+    arr = (TmcStack * count)()
+    d = dist if dist is not None else default_stack_dist()
+    code = load().tmc_generate_stacks(seed, first_id, count, C.byref(d), arr)
+    if code != TMC_OK:
+        raise ConfigError("invalid stack distribution")
+    return arr
+
+
+def stack_from_dict(st: dict) -> TmcStack:
+    s = TmcStack()
+    s.num_layers = len(st["layers"])
+    s.n_above, s.n_below = float(st["n_above"]), float(st["n_below"])
+    for i, L in enumerate(st["layers"]):
+        s.layers[i] = TmcLayer(L["n"], L["mua"], L["mus"], L["g"], L["d"])
+    s.photons = int(st["photons"])
+    return s
+
+
+def stack_to_dict(s: TmcStack) -> dict:
+    return {"n_above": s.n_above, "n_below": s.n_below, "photons": int(s.photons),
+            "layers": [{"n": s.layers[i].n, "mua": s.layers[i].mua, "mus": s.layers[i].mus,
+                        "g": s.layers[i].g, "d": s.layers[i].d} for i in range(s.num_layers)]}
+
+
+def stacks_array(stacks) -> C.Array:
+    if isinstance(stacks, C.Array):
+        return stacks
+    arr = (TmcStack * len(stacks))()
+    for i, st in enumerate(stacks):
+        arr[i] = st if isinstance(st, TmcStack) else stack_from_dict(st)
+    return arr
+
+
+class BatchedPlan:
+    """A resident batched plan: device buffers allocated once, clear + transport replayed as a
+    HIP graph on every execute()."""
+
+    def __init__(self, n_stacks: int, *, batches_per_stack: int = 10, dz: float = 0.01,
+                 dr: float = 0.01, nz: int = 100, nr: int = 100, device: int = 0,
+                 profiles: bool = True, graph: bool = True, waves_per_group: int = 0,
+                 groups: int = 0, chunk: int = 0, step_limit: int = 0, photons_hint: int = 0):
+        self.n_stacks, self.bs, self.nz, self.nr = n_stacks, batches_per_stack, nz, nr
+        o = TmcBatchedOpts(seed=1, batches_per_stack=batches_per_stack, device=device, dz=dz,
+                           dr=dr, nz=nz, nr=nr,
+                           flags=(0 if profiles else FLAG_NO_PROFILES) | (0 if graph else FLAG_NO_GRAPH),
+                           waves_per_group=waves_per_group, groups=groups, chunk=chunk,
+                           step_limit=step_limit, photons_hint=photons_hint)
+        self._h = C.c_void_p()
+        code = load().tmc_batched_create(n_stacks, C.byref(o), C.byref(self._h))
+        if code != TMC_OK:
+            _raise(code)
+
+    def execute(self, stacks, seed: int = 1) -> dict:
+        arr = stacks_array(stacks)
+        S, Bs, nz, nr = self.n_stacks, self.bs, self.nz, self.nr
+        rd, tt, ev, an, bs = (np.zeros(S * Bs, np.uint64) for _ in range(5))
+        ab = np.zeros(S * Bs * MAX_LAYERS, np.uint64)
+        az = np.zeros(S * nz, np.uint64)
+        rr = np.zeros(S * nr, np.uint64)
+        tr = np.zeros(S * nr, np.uint64)
+        out = TmcBatchedOut(rd=_p(rd), tt=_p(tt), absorbed=_p(ab), events=_p(ev), anomalies=_p(an),
+                            boundary_steps=_p(bs), az=_p(az), rr=_p(rr), tr=_p(tr))
+        code = load().tmc_batched_execute(self._h, arr, len(arr), seed, C.byref(out))
+        if code != TMC_OK:
+            _raise(code)
+        photons = np.array([arr[s].photons for s in range(len(arr))], np.uint64)
+        return {
+            "rd": rd.reshape(S, Bs), "tt": tt.reshape(S, Bs),
+            "absorbed": ab.reshape(S, Bs, MAX_LAYERS), "events": ev.reshape(S, Bs),
+            "anomalies": an.reshape(S, Bs), "boundary_steps": bs.reshape(S, Bs),
+            "az": az.reshape(S, nz), "rr": rr.reshape(S, nr), "tr": tr.reshape(S, nr),
+            "photons": photons, "kernel_seconds": out.kernel_seconds,
+            "wall_seconds": out.wall_seconds, "device": out.device.decode(),
+            "launch": {"groups": out.groups, "waves_per_group": out.waves_per_group,
+                       "lds_bytes": out.lds_bytes, "chunk": out.chunk},
+        }
+
+    def close(self) -> None:
+        if self._h:
+            load().tmc_batched_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_batched(stacks, *, seed: int = 1, **plan_kw) -> dict:
+    """One batched run (plan created and destroyed around a single execute)."""
+    arr = stacks_array(stacks)
+    with BatchedPlan(len(arr), **plan_kw) as plan:
+        return plan.execute(arr, seed)
+
+
+def batch_plan_rows(photons: int, batches: int) -> list[tuple[int, int]]:
+    """(first id, count) of each batch: the reference's BatchPlan (batching.hpp:11-29)."""
+    base, extra = divmod(photons, batches)
+    return [(b * base + min(b, extra), base + (1 if b < extra else 0)) for b in range(batches)]
