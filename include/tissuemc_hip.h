@@ -178,6 +178,49 @@ void tmc_default_stack_dist(tmc_stack_dist *dist);
 int tmc_generate_stacks(uint64_t seed, uint64_t first_id, int32_t count, const tmc_stack_dist *dist,
                         tmc_stack *out);
 
+/* ---------------------------------------------------------------- learned surrogate
+ *
+ * No reference counterpart. An ensemble of `heads` independent MLPs (inputs -> depth hidden
+ * layers of width `hidden` with GELU (tanh form) -> outputs), trained with Adam on a mean squared
+ * error, every matrix product in BF16 MFMA with FP32 accumulation. Parameters per head, in order
+ * per layer k: W_k [out_k x in_k] row-major, then b_k [out_k] (float32). */
+
+typedef struct tmc_mlp_config {
+    int32_t heads, inputs, hidden, depth, outputs;
+    int32_t batch;                  /* training batch per step and head (multiple of 64) */
+    float lr, beta1, beta2, eps, weight_decay;
+    uint64_t seed;                  /* initialisation and batch sampling */
+    int32_t device;
+} tmc_mlp_config;
+
+typedef struct tmc_mlp tmc_mlp; /* opaque */
+
+/* 4 heads, 30 inputs, 4 x 512 hidden, 202 outputs, batch 4096, Adam lr 1e-3 (0.9, 0.999, 1e-8). */
+void tmc_mlp_default_config(tmc_mlp_config *config);
+int tmc_mlp_create(const tmc_mlp_config *config, tmc_mlp **model);
+void tmc_mlp_destroy(tmc_mlp *model);
+/* Parameters per head (logical, unpadded). */
+int64_t tmc_mlp_param_count(const tmc_mlp *model);
+/* Upload the training set: x [n x inputs], y [n x outputs] (float32), loss weight per output
+ * (NULL = all 1). The set stays in device memory. */
+int tmc_mlp_set_data(tmc_mlp *model, const float *x, const float *y, int64_t n,
+                     const float *loss_weights);
+/* `steps` Adam steps on batches sampled from the set (each head its own samples); the step is a
+ * captured HIP graph replayed per step. losses: [steps x heads] or NULL; seconds: device time. */
+int tmc_mlp_train(tmc_mlp *model, int64_t steps, float *losses, double *seconds);
+/* Gradients of the loss on the given rows of the set (batch rows, no update): grads
+ * [heads x param_count], loss [heads]. */
+int tmc_mlp_gradients(tmc_mlp *model, const int64_t *rows, float *grads, float *loss);
+/* Ensemble prediction for x [n x inputs]: mean [n x outputs] over heads, spread [n x 2] (standard
+ * deviation over heads of outputs 0 and 1), per_head [heads x n x outputs] (any may be NULL).
+ * seconds: device time of the forward passes and reductions (transfers excluded). */
+int tmc_mlp_predict(tmc_mlp *model, const float *x, int64_t n, float *mean, float *spread,
+                    float *per_head, double *seconds);
+int tmc_mlp_get_params(tmc_mlp *model, float *params /* [heads x param_count] */);
+int tmc_mlp_set_params(tmc_mlp *model, const float *params);
+/* Algorithmic matrix-product FLOPs of one training step and of one query (all heads). */
+int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query);
+
 /* 1 if a HIP device is present and usable, else 0. */
 int tmc_hip_available(void);
 

@@ -31,8 +31,8 @@ comp.mco A # output filename, ASCII/Binary
 
 def declared_functions() -> list[str]:
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|double|void|const char \*)\s*(tmc_\w+)\s*\(", text,
-                                 flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|void|const char \*)\s*(tmc_\w+)\s*\(",
+                                 text, flags=re.M)))
 
 
 def test_library_exports_every_declared_function():

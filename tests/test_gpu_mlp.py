@@ -1,0 +1,125 @@
+"""The surrogate network's BF16 MFMA kernels against the float64 oracle (oracle/mlp_oracle.py).
+
+The GPU rounds weights, inputs and hidden activations to bfloat16 and accumulates in float32; the
+oracle emulates the bfloat16 rounding and accumulates in float64. Tolerances are written per
+check below."""
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(heads=2, inputs=30, hidden=128, depth=3, outputs=202, batch=128, seed=7)
+DIMS = (30, 128, 3, 202)
+
+
+@pytest.fixture()
+def model(gpu):
+    from tissuemc_amd.mlp import Mlp
+
+    m = Mlp(**CFG)
+    yield m
+    m.close()
+
+
+def data(n=512, seed=3):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (n, 30)).astype(np.float32)
+    y = rng.normal(0, 1, (n, 202)).astype(np.float32)
+    return x, y
+
+
+def test_param_count_and_flops(model):
+    assert model.param_count == M.param_count(*DIMS)
+    per_step, per_query = model.flops()
+    # padded shapes: in 30 -> 64, out 202 -> 256
+    q = 2 * (128 * 64 + 128 * 128 * 2 + 256 * 128) * 2
+    assert per_query == q
+
+
+def test_forward_matches_oracle(model):
+    x, _ = data(300)
+    mean, spread, ph, _ = model.predict(x, per_head=True)
+    params = model.get_params()
+    for h in range(2):
+        ref_bf, _ = M.forward(params[h].astype(np.float64), x.astype(np.float64), DIMS, bf16=True)
+        ref64, _ = M.forward(params[h].astype(np.float64), x.astype(np.float64), DIMS, bf16=False)
+        scale = np.abs(ref64).max()
+        # bf16 emulation: only float32 accumulation order and bf16 ties differ
+        assert np.max(np.abs(ph[h] - ref_bf)) <= 2e-2 * scale
+        assert np.linalg.norm(ph[h] - ref_bf) <= 2e-3 * np.linalg.norm(ref_bf)
+        # against exact float64: the bf16 operand rounding (2^-9 relative per operand)
+        assert np.linalg.norm(ph[h] - ref64) <= 2e-2 * np.linalg.norm(ref64)
+    assert np.allclose(mean, ph.mean(axis=0), rtol=1e-5, atol=1e-6)
+    assert np.allclose(spread, ph[:, :, :2].std(axis=0, ddof=1), rtol=1e-3, atol=1e-6)
+
+
+def test_gradients_match_oracle(model):
+    x, y = data(512)
+    w = np.linspace(0.5, 2.0, 202).astype(np.float32)
+    model.set_data(x, y, w)
+    rows = np.random.default_rng(5).integers(0, 512, 128)
+    g, loss = model.gradients(rows)
+    params = model.get_params()
+    for h in range(2):
+        ref_loss, ref = M.loss_and_grad(params[h].astype(np.float64), x[rows].astype(np.float64),
+                                        y[rows].astype(np.float64), w.astype(np.float64), DIMS,
+                                        bf16=True)
+        assert loss[h] == pytest.approx(ref_loss, rel=2e-3)
+        cos = float(g[h] @ ref / (np.linalg.norm(g[h]) * np.linalg.norm(ref)))
+        assert cos > 0.9995
+        assert np.linalg.norm(g[h] - ref) <= 3e-2 * np.linalg.norm(ref)
+        # per layer, weights and biases separately
+        k = 0
+        for o, i in M.shapes(*DIMS):
+            for n_el in (o * i, o):
+                a, b = g[h][k:k + n_el], ref[k:k + n_el]
+                assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b) + 1e-9
+                k += n_el
+
+
+def test_adam_steps_follow_closed_form(model):
+    """Every row of the set is the same sample, so any sampled batch has the same gradient; the
+    first Adam step is then p - lr * g / (|g| + eps), and the second follows the moment
+    recursion with the gradient at the new point."""
+    x, y = data(1)
+    model.set_data(np.repeat(x, 64, 0), np.repeat(y, 64, 0))
+    rows = np.zeros(128, np.int64)
+    p0 = model.get_params().astype(np.float64)
+    g0, _ = model.gradients(rows)
+    model.train(1)
+    p1 = model.get_params().astype(np.float64)
+    c = model.config
+    exp1, m, v = M.adam_step(p0, g0.astype(np.float64), 0.0, 0.0, 1, c.lr, c.beta1, c.beta2, c.eps)
+    assert np.allclose(p1, exp1, rtol=0, atol=2e-7 + 1e-6 * np.abs(p0).max())
+    g1, _ = model.gradients(rows)
+    model.train(1)
+    p2 = model.get_params().astype(np.float64)
+    exp2, _, _ = M.adam_step(p1, g1.astype(np.float64), m, v, 2, c.lr, c.beta1, c.beta2, c.eps)
+    assert np.allclose(p2, exp2, rtol=0, atol=2e-7 + 1e-6 * np.abs(p0).max())
+
+
+def test_training_learns_a_smooth_map_and_is_deterministic(gpu):
+    from tissuemc_amd.mlp import Mlp
+
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (8192, 30)).astype(np.float32)
+    a = rng.normal(0, 1, (30, 202)).astype(np.float32) / 6
+    y = np.sin(x @ a).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        with Mlp(**dict(CFG, batch=256, seed=11)) as m:
+            m.set_data(x, y)
+            losses, _ = m.train(600)
+            runs.append((losses, m.get_params()))
+    losses = runs[0][0]
+    assert losses[-50:].mean() < 0.1 * losses[:10].mean()
+    assert np.array_equal(runs[0][1], runs[1][1])  # same seed and data: bit-identical training
+
+
+def test_set_params_round_trip(model):
+    p = np.random.default_rng(1).normal(0, 0.1, (2, model.param_count)).astype(np.float32)
+    model.set_params(p)
+    assert np.array_equal(model.get_params(), p)

@@ -305,6 +305,115 @@ int tmc_hip_run_batched(const tmc_stack *stacks, int32_t n_stacks, const tmc_bat
     return rc;
 }
 
+struct tmc_mlp {
+    tmc::MlpModel *impl;
+};
+
+void tmc_mlp_default_config(tmc_mlp_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->heads = 4;
+    c->inputs = 30;
+    c->hidden = 512;
+    c->depth = 4;
+    c->outputs = 202;
+    c->batch = 4096;
+    c->lr = 1e-3f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.999f;
+    c->eps = 1e-8f;
+    c->weight_decay = 0.0f;
+    c->seed = 1;
+    c->device = 0;
+}
+
+int tmc_mlp_create(const tmc_mlp_config *config, tmc_mlp **model) {
+    return guarded([&] {
+        if (!config || !model) config_fail("config and model must not be NULL");
+        *model = nullptr;
+        auto *h = new tmc_mlp{nullptr};
+        try {
+            h->impl = tmc::mlp_create(*config);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *model = h;
+    });
+}
+
+void tmc_mlp_destroy(tmc_mlp *model) {
+    if (!model) return;
+    tmc::mlp_destroy(model->impl);
+    delete model;
+}
+
+int64_t tmc_mlp_param_count(const tmc_mlp *model) {
+    return model && model->impl ? tmc::mlp_param_count(*model->impl) : -1;
+}
+
+#define TMC_NEED_MODEL() \
+    if (!model || !model->impl) config_fail("model must not be NULL")
+
+int tmc_mlp_set_data(tmc_mlp *model, const float *x, const float *y, int64_t n, const float *w) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!x || !y) config_fail("x and y must not be NULL");
+        tmc::mlp_set_data(*model->impl, x, y, n, w);
+    });
+}
+
+int tmc_mlp_train(tmc_mlp *model, int64_t steps, float *losses, double *seconds) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        const double s = tmc::mlp_train(*model->impl, steps, losses);
+        if (seconds) *seconds = s;
+    });
+}
+
+int tmc_mlp_gradients(tmc_mlp *model, const int64_t *rows, float *grads, float *loss) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!rows || !grads) config_fail("rows and grads must not be NULL");
+        tmc::mlp_gradients(*model->impl, rows, grads, loss);
+    });
+}
+
+int tmc_mlp_predict(tmc_mlp *model, const float *x, int64_t n, float *mean, float *spread,
+                    float *per_head, double *seconds) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!x && n > 0) config_fail("x must not be NULL");
+        if (n < 0) config_fail("n must be >= 0");
+        const double s = tmc::mlp_predict(*model->impl, x, n, mean, spread, per_head);
+        if (seconds) *seconds = s;
+    });
+}
+
+int tmc_mlp_get_params(tmc_mlp *model, float *params) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!params) config_fail("params must not be NULL");
+        tmc::mlp_get_params(*model->impl, params);
+    });
+}
+
+int tmc_mlp_set_params(tmc_mlp *model, const float *params) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!params) config_fail("params must not be NULL");
+        tmc::mlp_set_params(*model->impl, params);
+    });
+}
+
+int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (per_step) *per_step = tmc::mlp_flops_per_step(*model->impl);
+        if (per_query) *per_query = tmc::mlp_flops_per_query(*model->impl);
+    });
+}
+
 int tmc_hip_philox_stream(uint64_t seed, uint64_t photon_id, uint32_t word3, int64_t n,
                           uint32_t *out, int32_t device) {
     return guarded([&] {

@@ -41,6 +41,20 @@ void batched_execute(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, u
                      tmc_batched_out *out);
 void batched_destroy(BatchedPlan *plan);
 
+// The surrogate model (mlp.hip).
+class MlpModel;
+MlpModel *mlp_create(const tmc_mlp_config &c);
+void mlp_destroy(MlpModel *m);
+void mlp_set_data(MlpModel &m, const float *x, const float *y, int64_t n, const float *w);
+double mlp_train(MlpModel &m, int64_t steps, float *losses);
+void mlp_gradients(MlpModel &m, const int64_t *rows, float *grads, float *loss);
+double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *spread, float *per_head);
+void mlp_get_params(MlpModel &m, float *out);
+void mlp_set_params(MlpModel &m, const float *in);
+long long mlp_param_count(const MlpModel &m);
+uint64_t mlp_flops_per_step(const MlpModel &m);
+uint64_t mlp_flops_per_query(const MlpModel &m);
+
 // Device-side Philox stream, for the parity hook.
 void philox_stream(uint64_t seed, uint64_t id, uint32_t word3, int64_t n, uint32_t *out, int device);
 

@@ -1,0 +1,887 @@
+// The learned surrogate: an ensemble of MLPs trained and evaluated with hand-written BF16 MFMA
+// kernels (FP32 accumulation) on gfx950. No reference counterpart.
+//
+// Network (per ensemble head): inputs -> depth x [hidden, GELU] -> outputs, GELU in its tanh
+// form. Every matrix product is one kernel, `gemm_nt`: C[m, n] = sum_k A[m, k] * B[n, k] with
+// both operands K-contiguous in memory (the layout the MFMA wants per lane), so the three
+// products of a layer read these layouts:
+//   forward          Z = H_prev . W^T     A = H_prev [batch x in],  B = Wt [out x in]
+//   input gradient   dH = dZ . W          A = dZ [batch x out],     B = W  [in x out]
+//   weight gradient  dW = dZ^T . H_prev   A = dZt [out x batch],    B = Ht_prev [in x batch]
+// so the forward epilogue writes activations in both orientations (H and Ht), the backward
+// epilogue writes dZ and dZt, and Adam keeps bf16 weights in both orientations (Wt and W).
+// Heads are independent networks batched along gridDim.z. Matrices are padded to multiples of
+// 64 with zeros that stay zero (zero inputs, zero loss weights).
+//
+// MFMA: v_mfma_f32_16x16x32_bf16. Lane l holds A row l%16 and B column l%16, k = 8*(l/16)..+7;
+// the 16x16 result has column l%16, rows 4*(l/16)..+3 (checked on the device). A workgroup of
+// four waves computes a 64x64 tile, each wave 32x32 (2x2 MFMA tiles), K in steps of 32, operands
+// loaded straight from global memory as 16-byte vectors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_rng.h"
+#include "engine.h"
+
+namespace tmc {
+namespace mlp {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef uint16_t bf16;  // storage type
+
+__host__ __device__ __forceinline__ bf16 to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return static_cast<bf16>(u >> 16);
+}
+__host__ __device__ __forceinline__ float from_bf16(bf16 h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float kGeluA = 0.044715f;
+
+__device__ __forceinline__ float tanh_fast(float u) {
+    // tanh(u) = 1 - 2 / (exp(2u) + 1), exp through v_exp_f32 (2^x)
+    const float e = __builtin_amdgcn_exp2f(2.0f * 1.4426950408889634f * u);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+__device__ __forceinline__ float gelu(float z) {
+    const float t = tanh_fast(kGeluC * (z + kGeluA * z * z * z));
+    return 0.5f * z * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float t = tanh_fast(kGeluC * (z + kGeluA * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+}
+
+enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kBwdWeight = 3 };
+
+struct GemmArgs {
+    const bf16 *A;
+    const bf16 *B;
+    int M, N, K, lda, ldb;
+    long long sA, sB;  // head strides (elements)
+    // epilogue operands (head strides in elements)
+    const float *bias;
+    long long sBias;
+    float *Cf;  // fp32 output [M x N] (Z for kFwdHidden, Y for kFwdOut, dW for kBwdWeight)
+    long long sCf;
+    bf16 *Cb;   // bf16 output [M x N] (H or dZ)
+    bf16 *Ct;   // bf16 transposed output [N x M] (Ht or dZt)
+    long long sCb;  // head stride of Cb and Ct (M * N)
+    const float *Zin;  // kBwdData: the pre-activation of this layer [M x N]
+    long long sZin;
+};
+
+template <int kEpi>
+__global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int h = static_cast<int>(blockIdx.z);
+    const int m0 = static_cast<int>(blockIdx.y) * 64 + (wave >> 1) * 32;
+    const int n0 = static_cast<int>(blockIdx.x) * 64 + (wave & 1) * 32;
+    const bf16 *A = g.A + h * g.sA;
+    const bf16 *B = g.B + h * g.sB;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const bf16 *a0 = A + static_cast<long long>(m0 + r) * g.lda + kq;
+    const bf16 *a1 = a0 + 16LL * g.lda;
+    const bf16 *b0 = B + static_cast<long long>(n0 + r) * g.ldb + kq;
+    const bf16 *b1 = b0 + 16LL * g.ldb;
+    for (int k = 0; k < g.K; k += 32) {
+        const bf16x8 fa0 = *reinterpret_cast<const bf16x8 *>(a0 + k);
+        const bf16x8 fa1 = *reinterpret_cast<const bf16x8 *>(a1 + k);
+        const bf16x8 fb0 = *reinterpret_cast<const bf16x8 *>(b0 + k);
+        const bf16x8 fb1 = *reinterpret_cast<const bf16x8 *>(b1 + k);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+    }
+
+    // Epilogue: lane holds rows mbase..mbase+3 of column n of each 16x16 tile.
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + 16 * j + r;
+            const int mb = m0 + 16 * i + (lane >> 4) * 4;
+            if (kEpi == kBwdWeight) {
+                float *C = g.Cf + h * g.sCf;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
+            } else if (kEpi == kFwdOut) {
+                float *C = g.Cf + h * g.sCf;
+                const float b = g.bias[h * g.sBias + n];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q] + b;
+            } else {
+                bf16 *Cb = g.Cb + h * g.sCb;
+                bf16 *Ct = g.Ct ? g.Ct + h * g.sCb : nullptr;
+                bf16 out[4];
+                if (kEpi == kFwdHidden) {
+                    float *Z = g.Cf ? g.Cf + h * g.sCf : nullptr;
+                    const float b = g.bias[h * g.sBias + n];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float z = acc[i][j][q] + b;
+                        if (Z) Z[static_cast<long long>(mb + q) * g.N + n] = z;
+                        out[q] = to_bf16(gelu(z));
+                    }
+                } else {  // kBwdData
+                    const float *Z = g.Zin + h * g.sZin;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        out[q] = to_bf16(acc[i][j][q] * gelu_grad(Z[static_cast<long long>(mb + q) * g.N + n]));
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Cb[static_cast<long long>(mb + q) * g.N + n] = out[q];
+                if (Ct) {
+                    uint2 packed;
+                    packed.x = static_cast<uint32_t>(out[0]) | (static_cast<uint32_t>(out[1]) << 16);
+                    packed.y = static_cast<uint32_t>(out[2]) | (static_cast<uint32_t>(out[3]) << 16);
+                    *reinterpret_cast<uint2 *>(Ct + static_cast<long long>(n) * g.M + mb) = packed;
+                }
+            }
+        }
+    }
+}
+
+// Batch assembly: head h, row m takes dataset sample idx (Philox of (step, m, h) mod n), writes
+// the standardized features padded to in_pad as bf16 X [B x in_pad] and Xt [in_pad x B], and the
+// targets padded to out_pad as fp32 Y [B x out_pad].
+__global__ void gather_batch(const float *xd, const float *yd, int64_t n, int in, int out, int in_pad,
+                             int out_pad, int B, const uint64_t *step, uint32_t key0, uint32_t key1,
+                             bf16 *X, bf16 *Xt, float *Y, const int64_t *fixed_rows) {
+    const int m = static_cast<int>(blockIdx.x);
+    const int h = static_cast<int>(blockIdx.y);
+    int64_t idx;
+    if (fixed_rows) {
+        idx = fixed_rows[m];
+    } else {
+        const Block4 v = philox4x32_10(static_cast<uint32_t>(m), static_cast<uint32_t>(h),
+                                       static_cast<uint32_t>(step[0]), 0x4D4C5031u, key0, key1);
+        idx = static_cast<int64_t>((static_cast<uint64_t>(v.v0) * static_cast<uint64_t>(n)) >> 32);
+    }
+    bf16 *Xh = X + static_cast<long long>(h) * B * in_pad;
+    bf16 *Xth = Xt + static_cast<long long>(h) * B * in_pad;
+    float *Yh = Y + static_cast<long long>(h) * B * out_pad;
+    for (int j = static_cast<int>(threadIdx.x); j < in_pad; j += static_cast<int>(blockDim.x)) {
+        const bf16 v = to_bf16(j < in ? xd[idx * in + j] : 0.0f);
+        Xh[static_cast<long long>(m) * in_pad + j] = v;
+        Xth[static_cast<long long>(j) * B + m] = v;
+    }
+    for (int j = static_cast<int>(threadIdx.x); j < out_pad; j += static_cast<int>(blockDim.x))
+        Yh[static_cast<long long>(m) * out_pad + j] = j < out ? yd[idx * out + j] : 0.0f;
+}
+
+// Weighted mean squared error over the real outputs; writes dY (bf16, both orientations) and
+// adds the batch loss of each head into loss[(step % ring) * heads + h].
+__global__ void loss_grad(const float *Yp, const float *Yt, const float *w, int B, int out_pad,
+                          float inv_count, bf16 *dY, bf16 *dYt, float *loss, const uint64_t *step,
+                          int heads, uint64_t ring) {
+    const int m = static_cast<int>(blockIdx.x);
+    const int h = static_cast<int>(blockIdx.y);
+    const long long base = (static_cast<long long>(h) * B + m) * out_pad;
+    float part = 0.0f;
+    for (int j = static_cast<int>(threadIdx.x); j < out_pad; j += static_cast<int>(blockDim.x)) {
+        const float d = Yp[base + j] - Yt[base + j];
+        const float wj = w[j];
+        part += wj * d * d;
+        const bf16 g = to_bf16(2.0f * wj * d * inv_count);
+        dY[base + j] = g;
+        dYt[(static_cast<long long>(h) * out_pad + j) * B + m] = g;
+    }
+    // wave64 reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63u) == 0 && loss)
+        atomicAdd(loss + static_cast<long long>(step[0] % ring) * heads + h, part * inv_count);
+}
+
+// Bias gradient: db[h][n] = sum_m dZt[h][n][m] (rows of the transposed gradient are contiguous).
+__global__ void bias_grad(const bf16 *dZt, int B, int N, float *db, long long s_db) {
+    const int n = static_cast<int>(blockIdx.x);
+    const int h = static_cast<int>(blockIdx.y);
+    const bf16 *row = dZt + (static_cast<long long>(h) * N + n) * B;
+    float s = 0.0f;
+    for (int m = static_cast<int>(threadIdx.x); m < B; m += static_cast<int>(blockDim.x)) s += from_bf16(row[m]);
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = static_cast<int>(blockDim.x) / 2; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[h * s_db + n] = red[0];
+}
+
+struct MatInfo {
+    long long off;  // offset of the matrix in a head's parameter block (floats)
+    int out, in;    // padded shape [out x in]
+    long long boff;  // offset of the bias
+    long long wt_off, w_off;  // offsets of the bf16 copies in a head's bf16 block
+};
+constexpr int kMaxMats = 8;
+
+struct AdamArgs {
+    float *p, *g, *m, *v;  // [heads x P]
+    bf16 *wb;              // [heads x Pb] bf16 copies (Wt then W per matrix)
+    long long P, Pb;
+    int n_mats;
+    MatInfo mats[kMaxMats];
+    float lr, beta1, beta2, eps, wd;
+    const uint64_t *step;
+};
+
+// Adam (Kingma & Ba 2015) with bias correction; also refreshes the bf16 weight copies.
+__global__ void adam_update(const AdamArgs a) {
+    const int h = static_cast<int>(blockIdx.y);
+    const float t = static_cast<float>(a.step[0] + 1);
+    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, t));
+    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, t));
+    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < a.P;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const float gr = g[e] + a.wd * p[e];
+        const float mn = a.beta1 * m[e] + (1.0f - a.beta1) * gr;
+        const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
+        m[e] = mn;
+        v[e] = vn;
+        const float pn = p[e] - a.lr * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
+        p[e] = pn;
+        for (int k = 0; k < a.n_mats; ++k) {
+            const MatInfo &mi = a.mats[k];
+            const long long rel = e - mi.off;
+            if (rel >= 0 && rel < static_cast<long long>(mi.out) * mi.in) {
+                const int o = static_cast<int>(rel / mi.in), i = static_cast<int>(rel % mi.in);
+                bf16 *wb = a.wb + h * a.Pb;
+                const bf16 b = to_bf16(pn);
+                wb[mi.wt_off + rel] = b;
+                wb[mi.w_off + static_cast<long long>(i) * mi.out + o] = b;
+            }
+        }
+    }
+}
+
+__global__ void bump_step(uint64_t *step) { step[0] += 1; }
+
+// Copies the bf16 weights from fp32 masters (after initialisation or set_params). Uses the
+// parameter and matrix fields of AdamArgs.
+__global__ void refresh_bf16(const AdamArgs a) {
+    const int h = static_cast<int>(blockIdx.y);
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < a.P;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        for (int k = 0; k < a.n_mats; ++k) {
+            const MatInfo &mi = a.mats[k];
+            const long long rel = e - mi.off;
+            if (rel >= 0 && rel < static_cast<long long>(mi.out) * mi.in) {
+                const int o = static_cast<int>(rel / mi.in), i = static_cast<int>(rel % mi.in);
+                const bf16 b = to_bf16(a.p[h * a.P + e]);
+                a.wb[h * a.Pb + mi.wt_off + rel] = b;
+                a.wb[h * a.Pb + mi.w_off + static_cast<long long>(i) * mi.out + o] = b;
+            }
+        }
+    }
+}
+
+}  // namespace mlp
+}  // namespace tmc
+
+namespace tmc {
+namespace mlp {
+
+// Inputs for prediction: the same rows for every head, bf16, zero-padded.
+__global__ void load_inputs(const float *xf, int rows, int in, int in_pad, int B, bf16 *X) {
+    const int m = static_cast<int>(blockIdx.x);
+    const int h = static_cast<int>(blockIdx.y);
+    for (int j = static_cast<int>(threadIdx.x); j < in_pad; j += static_cast<int>(blockDim.x)) {
+        const float v = (m < rows && j < in) ? xf[static_cast<long long>(m) * in + j] : 0.0f;
+        X[(static_cast<long long>(h) * B + m) * in_pad + j] = to_bf16(v);
+    }
+}
+
+// Ensemble mean of every output and the spread (standard deviation over heads) of the first
+// n_spread outputs, for rows [0, rows).
+__global__ void ensemble_reduce(const float *Yp, int heads, int B, int out_pad, int out, int n_spread,
+                                int rows, float *mean, float *spread) {
+    const int m = static_cast<int>(blockIdx.x);
+    if (m >= rows) return;
+    for (int j = static_cast<int>(threadIdx.x); j < out; j += static_cast<int>(blockDim.x)) {
+        float s = 0.0f;
+        for (int h = 0; h < heads; ++h) s += Yp[(static_cast<long long>(h) * B + m) * out_pad + j];
+        const float mu = s / heads;
+        mean[static_cast<long long>(m) * out + j] = mu;
+        if (j < n_spread) {  // two passes: no cancellation when the heads nearly agree
+            float ss = 0.0f;
+            for (int h = 0; h < heads; ++h) {
+                const float d = Yp[(static_cast<long long>(h) * B + m) * out_pad + j] - mu;
+                ss += d * d;
+            }
+            spread[static_cast<long long>(m) * n_spread + j] = heads > 1 ? __builtin_sqrtf(ss / (heads - 1)) : 0.0f;
+        }
+    }
+}
+
+}  // namespace mlp
+
+#define MLP_TRY(call)                                                                          \
+    do {                                                                                       \
+        const hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                                  \
+            throw EngineError(TMC_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+using mlp::bf16;
+
+class MlpModel {
+  public:
+    static int pad64(int x) { return (x + 63) / 64 * 64; }
+
+    explicit MlpModel(const tmc_mlp_config &c) : cfg_(c) {
+        if (c.heads < 1 || c.heads > 64) throw EngineError(TMC_ERR_CONFIG, "heads must be in [1, 64]");
+        if (c.depth < 1 || c.depth > mlp::kMaxMats - 1)
+            throw EngineError(TMC_ERR_CONFIG, "depth must be in [1, 7]");
+        if (c.hidden < 64 || c.hidden % 64) throw EngineError(TMC_ERR_CONFIG, "hidden must be a multiple of 64");
+        if (c.batch < 64 || c.batch % 64) throw EngineError(TMC_ERR_CONFIG, "batch must be a multiple of 64");
+        if (c.inputs < 1 || c.outputs < 1) throw EngineError(TMC_ERR_CONFIG, "inputs and outputs must be >= 1");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            throw EngineError(TMC_ERR_DEVICE, "no usable HIP device found");
+        if (c.device < 0 || c.device >= count) throw EngineError(TMC_ERR_CONFIG, "no such device");
+        MLP_TRY(hipSetDevice(c.device));
+        H_ = c.heads;
+        in_pad_ = pad64(c.inputs);
+        out_pad_ = pad64(c.outputs);
+        hid_ = c.hidden;
+        B_ = c.batch;
+        Bq_ = std::max(B_, 16384);
+        // parameter layout per head: W [out x in] then b [out], per matrix
+        long long off = 0, boff = 0;
+        for (int k = 0; k <= c.depth; ++k) {
+            mlp::MatInfo mi;
+            mi.in = k == 0 ? in_pad_ : hid_;
+            mi.out = k == c.depth ? out_pad_ : hid_;
+            mi.off = off;
+            off += static_cast<long long>(mi.out) * mi.in;
+            mi.boff = off;
+            off += mi.out;
+            mi.wt_off = boff;
+            boff += static_cast<long long>(mi.out) * mi.in;
+            mi.w_off = boff;
+            boff += static_cast<long long>(mi.out) * mi.in;
+            mats_.push_back(mi);
+            real_in_.push_back(k == 0 ? c.inputs : hid_);
+            real_out_.push_back(k == c.depth ? c.outputs : hid_);
+        }
+        P_ = off;
+        Pb_ = boff;
+        MLP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        MLP_TRY(hipEventCreate(&e0_));
+        MLP_TRY(hipEventCreate(&e1_));
+        p_ = alloc<float>(H_ * P_);
+        g_ = alloc<float>(H_ * P_);
+        m_ = alloc<float>(H_ * P_);
+        v_ = alloc<float>(H_ * P_);
+        wb_ = alloc<bf16>(H_ * Pb_);
+        step_ = alloc<uint64_t>(1);
+        lossw_ = alloc<float>(out_pad_);
+        loss_ = alloc<float>(kLossRing * H_);
+        const long long BH = static_cast<long long>(B_) * hid_;
+        X_ = alloc<bf16>(H_ * B_ * in_pad_);
+        Xt_ = alloc<bf16>(H_ * B_ * in_pad_);
+        Yt_ = alloc<float>(H_ * B_ * out_pad_);
+        Yp_ = alloc<float>(H_ * static_cast<long long>(Bq_) * out_pad_);
+        dY_ = alloc<bf16>(H_ * B_ * out_pad_);
+        dYt_ = alloc<bf16>(H_ * B_ * out_pad_);
+        for (int l = 0; l < c.depth; ++l) {
+            Z_.push_back(alloc<float>(H_ * BH));
+            Hs_.push_back(alloc<bf16>(H_ * BH));
+            Ht_.push_back(alloc<bf16>(H_ * BH));
+        }
+        for (int i = 0; i < 2; ++i) {
+            dZ_[i] = alloc<bf16>(H_ * BH);
+            dZt_[i] = alloc<bf16>(H_ * BH);
+        }
+        const long long QH = static_cast<long long>(Bq_) * hid_;
+        Xq_ = alloc<bf16>(H_ * static_cast<long long>(Bq_) * in_pad_);
+        Hq_[0] = alloc<bf16>(H_ * QH);
+        Hq_[1] = alloc<bf16>(H_ * QH);
+        xq_ = alloc<float>(static_cast<long long>(Bq_) * c.inputs);
+        mean_ = alloc<float>(static_cast<long long>(Bq_) * c.outputs);
+        spread_ = alloc<float>(static_cast<long long>(Bq_) * 2);
+        init_params();
+        std::vector<float> w(out_pad_, 0.0f);
+        for (int j = 0; j < c.outputs; ++j) w[j] = 1.0f;
+        MLP_TRY(hipMemcpy(lossw_, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        MLP_TRY(hipMemset(step_, 0, 8));
+        MLP_TRY(hipMemset(loss_, 0, static_cast<size_t>(kLossRing) * H_ * 4));
+    }
+
+    ~MlpModel() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (e0_) (void)hipEventDestroy(e0_);
+        if (e1_) (void)hipEventDestroy(e1_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+        for (void *q : allocs_) (void)hipFree(q);
+    }
+
+    long long logical_params() const {
+        long long n = 0;
+        for (size_t k = 0; k < mats_.size(); ++k)
+            n += static_cast<long long>(real_out_[k]) * real_in_[k] + real_out_[k];
+        return n;
+    }
+
+    void set_data(const float *x, const float *y, int64_t n, const float *loss_w) {
+        if (n < 1) throw EngineError(TMC_ERR_CONFIG, "the training set is empty");
+        if (xd_) (void)hipFree(xd_);
+        if (yd_) (void)hipFree(yd_);
+        xd_ = yd_ = nullptr;
+        MLP_TRY(hipMalloc(&xd_, static_cast<size_t>(n) * cfg_.inputs * 4));
+        MLP_TRY(hipMalloc(&yd_, static_cast<size_t>(n) * cfg_.outputs * 4));
+        MLP_TRY(hipMemcpy(xd_, x, static_cast<size_t>(n) * cfg_.inputs * 4, hipMemcpyHostToDevice));
+        MLP_TRY(hipMemcpy(yd_, y, static_cast<size_t>(n) * cfg_.outputs * 4, hipMemcpyHostToDevice));
+        n_data_ = n;
+        if (loss_w) {
+            std::vector<float> w(out_pad_, 0.0f);
+            for (int j = 0; j < cfg_.outputs; ++j) w[j] = loss_w[j];
+            MLP_TRY(hipMemcpy(lossw_, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        }
+        if (graph_exec_) {  // the data pointers are baked into the captured step
+            (void)hipGraphExecDestroy(graph_exec_);
+            (void)hipGraphDestroy(graph_);
+            graph_exec_ = nullptr;
+            graph_ = nullptr;
+        }
+    }
+
+    // Run `steps` optimisation steps (one captured graph per step, replayed). Returns seconds.
+    double train(int64_t steps, float *losses) {
+        if (!xd_) throw EngineError(TMC_ERR_CONFIG, "no training data (tmc_mlp_set_data)");
+        if (steps < 0 || steps > kLossRing) throw EngineError(TMC_ERR_CONFIG, "steps must be in [0, 65536]");
+        uint64_t step0 = 0;
+        MLP_TRY(hipMemcpy(&step0, step_, 8, hipMemcpyDeviceToHost));
+        if (!graph_exec_) {
+            MLP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            enqueue_step(nullptr, true);
+            MLP_TRY(hipStreamEndCapture(stream_, &graph_));
+            MLP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        }
+        MLP_TRY(hipEventRecord(e0_, stream_));
+        for (int64_t s = 0; s < steps; ++s) MLP_TRY(hipGraphLaunch(graph_exec_, stream_));
+        MLP_TRY(hipEventRecord(e1_, stream_));
+        MLP_TRY(hipEventSynchronize(e1_));
+        float ms = 0.0f;
+        MLP_TRY(hipEventElapsedTime(&ms, e0_, e1_));
+        if (losses && steps > 0) {
+            std::vector<float> ring(static_cast<size_t>(kLossRing) * H_);
+            MLP_TRY(hipMemcpy(ring.data(), loss_, ring.size() * 4, hipMemcpyDeviceToHost));
+            for (int64_t s = 0; s < steps; ++s) {
+                const int64_t slot = static_cast<int64_t>((step0 + s) % kLossRing);
+                for (int h = 0; h < H_; ++h) losses[s * H_ + h] = ring[slot * H_ + h];
+            }
+        }
+        return ms / 1000.0;
+    }
+
+    // One forward/backward on the given dataset rows (no update): gradients [heads x logical].
+    void gradients(const int64_t *rows, float *grads, float *loss) {
+        if (!xd_) throw EngineError(TMC_ERR_CONFIG, "no training data (tmc_mlp_set_data)");
+        for (int i = 0; i < B_; ++i)
+            if (rows[i] < 0 || rows[i] >= n_data_)
+                throw EngineError(TMC_ERR_CONFIG, "row " + std::to_string(rows[i]) + " is outside the training set");
+        int64_t *drows = nullptr;
+        MLP_TRY(hipMalloc(&drows, static_cast<size_t>(B_) * 8));
+        MLP_TRY(hipMemcpy(drows, rows, static_cast<size_t>(B_) * 8, hipMemcpyHostToDevice));
+        uint64_t step0 = 0;
+        MLP_TRY(hipMemcpy(&step0, step_, 8, hipMemcpyDeviceToHost));
+        const int64_t slot = static_cast<int64_t>(step0 % kLossRing);
+        MLP_TRY(hipMemsetAsync(loss_ + slot * H_, 0, static_cast<size_t>(H_) * 4, stream_));
+        enqueue_step(drows, false);
+        MLP_TRY(hipStreamSynchronize(stream_));
+        (void)hipFree(drows);
+        std::vector<float> g(static_cast<size_t>(H_) * P_);
+        MLP_TRY(hipMemcpy(g.data(), g_, g.size() * 4, hipMemcpyDeviceToHost));
+        unpad(g.data(), grads);
+        if (loss) {
+            std::vector<float> l(H_);
+            MLP_TRY(hipMemcpy(l.data(), loss_ + slot * H_, l.size() * 4, hipMemcpyDeviceToHost));
+            for (int h = 0; h < H_; ++h) loss[h] = l[h];
+        }
+    }
+
+    // Ensemble prediction: mean of every output and the spread of the first two. Returns the
+    // device time (upload excluded, reductions included) in seconds.
+    double predict(const float *x, int64_t n, float *mean, float *spread, float *per_head) {
+        double total = 0.0;
+        for (int64_t r0 = 0; r0 < n; r0 += Bq_) {
+            const int rows = static_cast<int>(std::min<int64_t>(Bq_, n - r0));
+            MLP_TRY(hipMemcpyAsync(xq_, x + r0 * cfg_.inputs, static_cast<size_t>(rows) * cfg_.inputs * 4,
+                                   hipMemcpyHostToDevice, stream_));
+            MLP_TRY(hipEventRecord(e0_, stream_));
+            hipLaunchKernelGGL(mlp::load_inputs, dim3(Bq_, H_), dim3(64), 0, stream_, xq_, rows,
+                               cfg_.inputs, in_pad_, Bq_, Xq_);
+            forward(Xq_, Bq_, Hq_, nullptr, nullptr, true);
+            hipLaunchKernelGGL(mlp::ensemble_reduce, dim3(rows), dim3(256), 0, stream_, Yp_, H_, Bq_,
+                               out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
+            MLP_TRY(hipGetLastError());
+            MLP_TRY(hipEventRecord(e1_, stream_));
+            if (mean)
+                MLP_TRY(hipMemcpyAsync(mean + r0 * cfg_.outputs, mean_, static_cast<size_t>(rows) * cfg_.outputs * 4,
+                                       hipMemcpyDeviceToHost, stream_));
+            if (spread)
+                MLP_TRY(hipMemcpyAsync(spread + r0 * 2, spread_, static_cast<size_t>(rows) * 2 * 4,
+                                       hipMemcpyDeviceToHost, stream_));
+            if (per_head)
+                for (int h = 0; h < H_; ++h)
+                    MLP_TRY(hipMemcpy2DAsync(per_head + (static_cast<int64_t>(h) * n + r0) * cfg_.outputs,
+                                             cfg_.outputs * 4, Yp_ + static_cast<long long>(h) * Bq_ * out_pad_,
+                                             out_pad_ * 4, cfg_.outputs * 4, rows, hipMemcpyDeviceToHost, stream_));
+            MLP_TRY(hipStreamSynchronize(stream_));
+            float ms = 0.0f;
+            MLP_TRY(hipEventElapsedTime(&ms, e0_, e1_));
+            total += ms / 1000.0;
+        }
+        return total;
+    }
+
+    void get_params(float *out) {
+        std::vector<float> p(static_cast<size_t>(H_) * P_);
+        MLP_TRY(hipMemcpy(p.data(), p_, p.size() * 4, hipMemcpyDeviceToHost));
+        unpad(p.data(), out);
+    }
+
+    void set_params(const float *in) {
+        std::vector<float> p(static_cast<size_t>(H_) * P_, 0.0f);
+        pad(in, p.data());
+        MLP_TRY(hipMemcpy(p_, p.data(), p.size() * 4, hipMemcpyHostToDevice));
+        refresh_bf16();
+    }
+
+    int heads() const { return H_; }
+    uint64_t flops_per_step() const {
+        uint64_t f = 0;
+        for (size_t k = 0; k < mats_.size(); ++k)
+            f += 2ull * B_ * mats_[k].out * mats_[k].in * (k == 0 ? 2 : 3);  // fwd + dW (+ dX)
+        return f * H_;
+    }
+    uint64_t flops_per_query() const {
+        uint64_t f = 0;
+        for (const auto &mi : mats_) f += 2ull * mi.out * mi.in;
+        return f * H_;
+    }
+
+  private:
+    static constexpr int kLossRing = 65536;
+
+    template <class T> T *alloc(long long n) {
+        void *q = nullptr;
+        MLP_TRY(hipMalloc(&q, static_cast<size_t>(std::max<long long>(n, 1)) * sizeof(T)));
+        MLP_TRY(hipMemset(q, 0, static_cast<size_t>(std::max<long long>(n, 1)) * sizeof(T)));
+        allocs_.push_back(q);
+        return static_cast<T *>(q);
+    }
+
+    // Logical (unpadded) <-> padded parameter layouts; per head: W [out x in] then b [out].
+    void unpad(const float *padded, float *logical) const {
+        long long o = 0;
+        for (int h = 0; h < H_; ++h)
+            for (size_t k = 0; k < mats_.size(); ++k) {
+                const auto &mi = mats_[k];
+                const float *src = padded + h * P_;
+                for (int r = 0; r < real_out_[k]; ++r)
+                    for (int c = 0; c < real_in_[k]; ++c) logical[o++] = src[mi.off + static_cast<long long>(r) * mi.in + c];
+                for (int r = 0; r < real_out_[k]; ++r) logical[o++] = src[mi.boff + r];
+            }
+    }
+    void pad(const float *logical, float *padded) const {
+        long long o = 0;
+        for (int h = 0; h < H_; ++h)
+            for (size_t k = 0; k < mats_.size(); ++k) {
+                const auto &mi = mats_[k];
+                float *dst = padded + h * P_;
+                for (int r = 0; r < real_out_[k]; ++r)
+                    for (int c = 0; c < real_in_[k]; ++c) dst[mi.off + static_cast<long long>(r) * mi.in + c] = logical[o++];
+                for (int r = 0; r < real_out_[k]; ++r) dst[mi.boff + r] = logical[o++];
+            }
+    }
+
+    // He-style initialisation from Philox (seed, head, element): uniform in +-sqrt(6 / fan_in).
+    void init_params() {
+        std::vector<float> p(static_cast<size_t>(H_) * P_, 0.0f);
+        const uint32_t k0 = static_cast<uint32_t>(cfg_.seed), k1 = static_cast<uint32_t>(cfg_.seed >> 32);
+        for (int h = 0; h < H_; ++h)
+            for (size_t k = 0; k < mats_.size(); ++k) {
+                const auto &mi = mats_[k];
+                const float a = std::sqrt(6.0f / static_cast<float>(real_in_[k]));
+                for (int r = 0; r < real_out_[k]; ++r)
+                    for (int c = 0; c < real_in_[k]; ++c) {
+                        const uint64_t e = static_cast<uint64_t>(r) * mi.in + c;
+                        const Block4 v = philox4x32_10(static_cast<uint32_t>(e), static_cast<uint32_t>(k),
+                                                       static_cast<uint32_t>(h), 0x494E4954u, k0, k1);
+                        const float u = (static_cast<float>(v.v0 >> 8) + 0.5f) * 0x1p-24f;
+                        p[static_cast<size_t>(h) * P_ + mi.off + e] = a * (2.0f * u - 1.0f);
+                    }
+            }
+        MLP_TRY(hipMemcpy(p_, p.data(), p.size() * 4, hipMemcpyHostToDevice));
+        refresh_bf16();
+    }
+
+    void refresh_bf16() {
+        mlp::AdamArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.p = p_;
+        a.wb = wb_;
+        a.P = P_;
+        a.Pb = Pb_;
+        a.n_mats = static_cast<int>(mats_.size());
+        for (size_t k = 0; k < mats_.size(); ++k) a.mats[k] = mats_[k];
+        hipLaunchKernelGGL(mlp::refresh_bf16, dim3(256, H_), dim3(256), 0, stream_, a);
+        MLP_TRY(hipGetLastError());
+        MLP_TRY(hipStreamSynchronize(stream_));
+    }
+
+    template <int kEpi> void gemm(const mlp::GemmArgs &g) {
+        if (g.M % 64 || g.N % 64 || g.K % 32)
+            throw EngineError(TMC_ERR_RUNTIME, "gemm_nt shape is not a multiple of the tile");
+        hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 64, g.M / 64, H_), dim3(256), 0, stream_, g);
+        MLP_TRY(hipGetLastError());
+    }
+
+    mlp::GemmArgs base_args() const {
+        mlp::GemmArgs g;
+        std::memset(&g, 0, sizeof g);
+        return g;
+    }
+
+    // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
+    // alternate between two buffers and neither Z nor Ht is kept.
+    void forward(bf16 *X, int Bn, bf16 *const *Hbuf, float *const *Zbuf, bf16 *const *Htbuf, bool infer) {
+        const long long BH = static_cast<long long>(Bn) * hid_;
+        const bf16 *in = X;
+        int in_dim = in_pad_;
+        for (int l = 0; l < cfg_.depth; ++l) {
+            const auto &mi = mats_[l];
+            mlp::GemmArgs g = base_args();
+            g.A = in;
+            g.lda = in_dim;
+            g.sA = static_cast<long long>(Bn) * in_dim;
+            g.B = wb_ + mi.wt_off;
+            g.ldb = mi.in;
+            g.sB = Pb_;
+            g.M = Bn;
+            g.N = hid_;
+            g.K = in_dim;
+            g.bias = p_ + mi.boff;
+            g.sBias = P_;
+            g.Cf = infer ? nullptr : Zbuf[l];
+            g.sCf = BH;
+            g.Cb = infer ? Hbuf[l % 2] : Hbuf[l];
+            g.Ct = infer ? nullptr : Htbuf[l];
+            g.sCb = BH;
+            gemm<mlp::kFwdHidden>(g);
+            in = g.Cb;
+            in_dim = hid_;
+        }
+        const auto &mo = mats_[cfg_.depth];
+        mlp::GemmArgs g = base_args();
+        g.A = in;
+        g.lda = hid_;
+        g.sA = BH;
+        g.B = wb_ + mo.wt_off;
+        g.ldb = hid_;
+        g.sB = Pb_;
+        g.M = Bn;
+        g.N = out_pad_;
+        g.K = hid_;
+        g.bias = p_ + mo.boff;
+        g.sBias = P_;
+        g.Cf = Yp_;
+        g.sCf = static_cast<long long>(Bn) * out_pad_;
+        gemm<mlp::kFwdOut>(g);
+    }
+
+    // One training step (gather, forward, loss, backward, and Adam if `update`).
+    void enqueue_step(const int64_t *fixed_rows, bool update) {
+        const uint32_t k0 = static_cast<uint32_t>(cfg_.seed ^ 0x5DEECE66DULL);
+        const uint32_t k1 = static_cast<uint32_t>((cfg_.seed >> 32) ^ 0xB5297A4DULL);
+        hipLaunchKernelGGL(mlp::gather_batch, dim3(B_, H_), dim3(64), 0, stream_, xd_, yd_, n_data_,
+                           cfg_.inputs, cfg_.outputs, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
+                           Yt_, fixed_rows);
+        MLP_TRY(hipGetLastError());
+        forward(X_, B_, Hs_.data(), Z_.data(), Ht_.data(), false);
+        const float inv_count = 1.0f / (static_cast<float>(B_) * cfg_.outputs);
+        hipLaunchKernelGGL(mlp::loss_grad, dim3(B_, H_), dim3(64), 0, stream_, Yp_, Yt_, lossw_, B_,
+                           out_pad_, inv_count, dY_, dYt_, loss_, step_, H_,
+                           static_cast<uint64_t>(kLossRing));
+        MLP_TRY(hipGetLastError());
+        const long long BH = static_cast<long long>(B_) * hid_;
+        const int D = cfg_.depth;
+        // output layer
+        {
+            const auto &mo = mats_[D];
+            mlp::GemmArgs g = base_args();
+            g.A = dYt_;
+            g.lda = B_;
+            g.sA = static_cast<long long>(out_pad_) * B_;
+            g.B = Ht_[D - 1];
+            g.ldb = B_;
+            g.sB = BH;
+            g.M = out_pad_;
+            g.N = hid_;
+            g.K = B_;
+            g.Cf = g_ + mo.off;
+            g.sCf = P_;
+            gemm<mlp::kBwdWeight>(g);
+            hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_, H_), dim3(256), 0, stream_, dYt_, B_,
+                               out_pad_, g_ + mo.boff, P_);
+            MLP_TRY(hipGetLastError());
+            mlp::GemmArgs d = base_args();
+            d.A = dY_;
+            d.lda = out_pad_;
+            d.sA = static_cast<long long>(B_) * out_pad_;
+            d.B = wb_ + mo.w_off;
+            d.ldb = out_pad_;
+            d.sB = Pb_;
+            d.M = B_;
+            d.N = hid_;
+            d.K = out_pad_;
+            d.Zin = Z_[D - 1];
+            d.sZin = BH;
+            d.Cb = dZ_[0];
+            d.Ct = dZt_[0];
+            d.sCb = BH;
+            gemm<mlp::kBwdData>(d);
+        }
+        int cur = 0;
+        for (int l = D - 1; l >= 0; --l) {
+            const auto &mi = mats_[l];
+            mlp::GemmArgs g = base_args();
+            g.A = dZt_[cur];
+            g.lda = B_;
+            g.sA = BH;
+            g.B = l == 0 ? Xt_ : Ht_[l - 1];
+            g.ldb = B_;
+            g.sB = l == 0 ? static_cast<long long>(B_) * in_pad_ : BH;
+            g.M = hid_;
+            g.N = mi.in;
+            g.K = B_;
+            g.Cf = g_ + mi.off;
+            g.sCf = P_;
+            gemm<mlp::kBwdWeight>(g);
+            hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
+                               g_ + mi.boff, P_);
+            MLP_TRY(hipGetLastError());
+            if (l > 0) {
+                mlp::GemmArgs d = base_args();
+                d.A = dZ_[cur];
+                d.lda = hid_;
+                d.sA = BH;
+                d.B = wb_ + mi.w_off;
+                d.ldb = hid_;
+                d.sB = Pb_;
+                d.M = B_;
+                d.N = hid_;
+                d.K = hid_;
+                d.Zin = Z_[l - 1];
+                d.sZin = BH;
+                d.Cb = dZ_[1 - cur];
+                d.Ct = dZt_[1 - cur];
+                d.sCb = BH;
+                gemm<mlp::kBwdData>(d);
+                cur = 1 - cur;
+            }
+        }
+        if (update) {
+            mlp::AdamArgs a;
+            std::memset(&a, 0, sizeof a);
+            a.p = p_;
+            a.g = g_;
+            a.m = m_;
+            a.v = v_;
+            a.wb = wb_;
+            a.P = P_;
+            a.Pb = Pb_;
+            a.n_mats = static_cast<int>(mats_.size());
+            for (size_t k = 0; k < mats_.size(); ++k) a.mats[k] = mats_[k];
+            a.lr = cfg_.lr;
+            a.beta1 = cfg_.beta1;
+            a.beta2 = cfg_.beta2;
+            a.eps = cfg_.eps;
+            a.wd = cfg_.weight_decay;
+            a.step = step_;
+            hipLaunchKernelGGL(mlp::adam_update, dim3(512, H_), dim3(256), 0, stream_, a);
+            MLP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(mlp::bump_step, dim3(1), dim3(1), 0, stream_, step_);
+            MLP_TRY(hipGetLastError());
+        }
+    }
+
+    tmc_mlp_config cfg_;
+    int H_ = 1, in_pad_ = 64, out_pad_ = 64, hid_ = 64, B_ = 64, Bq_ = 64;
+    long long P_ = 0, Pb_ = 0;
+    std::vector<mlp::MatInfo> mats_;
+    std::vector<int> real_in_, real_out_;
+    std::vector<void *> allocs_;
+    float *p_ = nullptr, *g_ = nullptr, *m_ = nullptr, *v_ = nullptr, *lossw_ = nullptr, *loss_ = nullptr;
+    bf16 *wb_ = nullptr;
+    uint64_t *step_ = nullptr;
+    bf16 *X_ = nullptr, *Xt_ = nullptr, *dY_ = nullptr, *dYt_ = nullptr;
+    float *Yt_ = nullptr, *Yp_ = nullptr;
+    std::vector<float *> Z_;
+    std::vector<bf16 *> Hs_, Ht_;
+    bf16 *dZ_[2] = {nullptr, nullptr}, *dZt_[2] = {nullptr, nullptr};
+    bf16 *Xq_ = nullptr, *Hq_[2] = {nullptr, nullptr};
+    float *xq_ = nullptr, *mean_ = nullptr, *spread_ = nullptr;
+    float *xd_ = nullptr, *yd_ = nullptr;
+    int64_t n_data_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+
+  public:
+    void free_data() {
+        if (xd_) (void)hipFree(xd_);
+        if (yd_) (void)hipFree(yd_);
+        xd_ = yd_ = nullptr;
+    }
+};
+
+MlpModel *mlp_create(const tmc_mlp_config &c) { return new MlpModel(c); }
+void mlp_destroy(MlpModel *m) {
+    if (m) m->free_data();
+    delete m;
+}
+void mlp_set_data(MlpModel &m, const float *x, const float *y, int64_t n, const float *w) { m.set_data(x, y, n, w); }
+double mlp_train(MlpModel &m, int64_t steps, float *losses) { return m.train(steps, losses); }
+void mlp_gradients(MlpModel &m, const int64_t *rows, float *grads, float *loss) { m.gradients(rows, grads, loss); }
+double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *spread, float *per_head) {
+    return m.predict(x, n, mean, spread, per_head);
+}
+void mlp_get_params(MlpModel &m, float *out) { m.get_params(out); }
+void mlp_set_params(MlpModel &m, const float *in) { m.set_params(in); }
+long long mlp_param_count(const MlpModel &m) { return m.logical_params(); }
+uint64_t mlp_flops_per_step(const MlpModel &m) { return m.flops_per_step(); }
+uint64_t mlp_flops_per_query(const MlpModel &m) { return m.flops_per_query(); }
+
+}  // namespace tmc

@@ -1,0 +1,149 @@
+"""Python binding of the surrogate network's C ABI (tmc_mlp_*, include/tissuemc_hip.h).
+
+An ensemble of MLPs trained and evaluated by the library's BF16 MFMA kernels. Parameters per
+head, per layer: W [out x in] row-major, then b [out], float32.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import engine
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [("heads", C.c_int32), ("inputs", C.c_int32), ("hidden", C.c_int32),
+                ("depth", C.c_int32), ("outputs", C.c_int32), ("batch", C.c_int32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("weight_decay", C.c_float), ("seed", C.c_uint64), ("device", C.c_int32)]
+
+
+_PF = C.POINTER(C.c_float)
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = engine.load()
+    if not _bound:
+        L.tmc_mlp_default_config.argtypes = [C.POINTER(MlpConfig)]
+        L.tmc_mlp_create.argtypes = [C.POINTER(MlpConfig), C.POINTER(C.c_void_p)]
+        L.tmc_mlp_create.restype = C.c_int
+        L.tmc_mlp_destroy.argtypes = [C.c_void_p]
+        L.tmc_mlp_destroy.restype = None
+        L.tmc_mlp_param_count.argtypes = [C.c_void_p]
+        L.tmc_mlp_param_count.restype = C.c_int64
+        L.tmc_mlp_set_data.argtypes = [C.c_void_p, _PF, _PF, C.c_int64, _PF]
+        L.tmc_mlp_set_data.restype = C.c_int
+        L.tmc_mlp_train.argtypes = [C.c_void_p, C.c_int64, _PF, C.POINTER(C.c_double)]
+        L.tmc_mlp_train.restype = C.c_int
+        L.tmc_mlp_gradients.argtypes = [C.c_void_p, C.POINTER(C.c_int64), _PF, _PF]
+        L.tmc_mlp_gradients.restype = C.c_int
+        L.tmc_mlp_predict.argtypes = [C.c_void_p, _PF, C.c_int64, _PF, _PF, _PF,
+                                      C.POINTER(C.c_double)]
+        L.tmc_mlp_predict.restype = C.c_int
+        L.tmc_mlp_get_params.argtypes = [C.c_void_p, _PF]
+        L.tmc_mlp_get_params.restype = C.c_int
+        L.tmc_mlp_set_params.argtypes = [C.c_void_p, _PF]
+        L.tmc_mlp_set_params.restype = C.c_int
+        L.tmc_mlp_flops.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.tmc_mlp_flops.restype = C.c_int
+        _bound = True
+    return L
+
+
+def _f(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_PF)
+
+
+def default_config(**overrides) -> MlpConfig:
+    c = MlpConfig()
+    _lib().tmc_mlp_default_config(C.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class Mlp:
+    def __init__(self, **config):
+        self.config = default_config(**config)
+        self._h = C.c_void_p()
+        code = _lib().tmc_mlp_create(C.byref(self.config), C.byref(self._h))
+        if code != engine.TMC_OK:
+            engine._raise(code)
+        self.heads = self.config.heads
+        self.inputs, self.outputs = self.config.inputs, self.config.outputs
+        self.param_count = int(_lib().tmc_mlp_param_count(self._h))
+
+    def _check(self, code: int) -> None:
+        if code != engine.TMC_OK:
+            engine._raise(code)
+
+    def set_data(self, x: np.ndarray, y: np.ndarray, loss_weights: np.ndarray | None = None) -> None:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        w = None if loss_weights is None else np.ascontiguousarray(loss_weights, np.float32)
+        assert x.shape[1] == self.inputs and y.shape[1] == self.outputs and len(x) == len(y)
+        self._check(_lib().tmc_mlp_set_data(self._h, _f(x), _f(y), len(x), _f(w)))
+
+    def train(self, steps: int) -> tuple[np.ndarray, float]:
+        """Run Adam steps; returns (losses [steps x heads], device seconds)."""
+        losses = np.zeros((steps, self.heads), np.float32)
+        sec = C.c_double(0)
+        self._check(_lib().tmc_mlp_train(self._h, steps, _f(losses), C.byref(sec)))
+        return losses, sec.value
+
+    def gradients(self, rows: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        rows = np.ascontiguousarray(rows, np.int64)
+        assert len(rows) == self.config.batch
+        g = np.zeros((self.heads, self.param_count), np.float32)
+        loss = np.zeros(self.heads, np.float32)
+        self._check(_lib().tmc_mlp_gradients(self._h, rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             _f(g), _f(loss)))
+        return g, loss
+
+    def predict(self, x: np.ndarray, per_head: bool = False):
+        """(mean [n x outputs], spread [n x 2], per-head [heads x n x outputs] or None, seconds)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n = len(x)
+        mean = np.zeros((n, self.outputs), np.float32)
+        spread = np.zeros((n, 2), np.float32)
+        ph = np.zeros((self.heads, n, self.outputs), np.float32) if per_head else None
+        sec = C.c_double(0)
+        self._check(_lib().tmc_mlp_predict(self._h, _f(x), n, _f(mean), _f(spread), _f(ph),
+                                           C.byref(sec)))
+        return mean, spread, ph, sec.value
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros((self.heads, self.param_count), np.float32)
+        self._check(_lib().tmc_mlp_get_params(self._h, _f(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (self.heads, self.param_count)
+        self._check(_lib().tmc_mlp_set_params(self._h, _f(p)))
+
+    def flops(self) -> tuple[int, int]:
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        self._check(_lib().tmc_mlp_flops(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self) -> None:
+        if self._h:
+            _lib().tmc_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
