@@ -122,6 +122,47 @@ def bench_batched(engine, steps: int, warmup: int) -> dict:
             "launch": res[0]["launch"]}
 
 
+def bench_surrogate() -> dict:
+    """The surrogate's kernels at full size: ensemble inference on 1.05e6 queries and a burst
+    of training steps (4 heads, 30 -> 4 x 512 GELU -> 202, batch 4096), device time by HIP
+    events; errors and the matched-error comparison come from the full pipeline run recorded in
+    profiles/surrogate_pipeline_full.json."""
+    import numpy as np
+
+    from tissuemc_amd.mlp import Mlp
+
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (1_050_000, 30)).astype(np.float32)
+    with Mlp() as net:
+        net.predict(x[:20000])  # warm-up
+        _, _, _, sec = net.predict(x)
+        per_step, per_query = net.flops()
+        net.set_data(x[:65536], rng.normal(0, 1, (65536, 202)).astype(np.float32))
+        net.train(20)
+        _, tsec = net.train(200)
+    out = {"inference_queries": len(x), "inference_device_s": sec,
+           "inference_stacks_per_s": len(x) / sec,
+           "inference_tflops": per_query * len(x) / sec / 1e12,
+           "inference_mfma_frac": per_query * len(x) / sec / 2.5e15,
+           "train_step_ms": tsec / 200 * 1e3, "train_tflops": per_step * 200 / tsec / 1e12,
+           "train_mfma_frac": per_step * 200 / tsec / 2.5e15}
+    rep = ROOT / "profiles" / "surrogate_pipeline_full.json"
+    if rep.exists():
+        r = json.loads(rep.read_text())
+        m = r["matched"]
+        out["recorded_pipeline"] = {
+            "source": "profiles/surrogate_pipeline_full.json",
+            "accepted_rd_median_rel_err": r["judge"]["rd_surrogate_accepted"]["median"],
+            "accepted_tt_median_rel_err": r["judge"]["tt_surrogate_accepted"]["median"],
+            "routed_frac": r["inference"]["routed_frac"],
+            "mc_stacks_per_s_at_matched_err": m["mc_stacks_per_s_matched"],
+            "speedup_in_domain_recorded": m["speedup_in_domain"],
+            "speedup_with_fallback_recorded": m["speedup_with_fallback"]}
+        out["speedup_in_domain_live"] = out["inference_stacks_per_s"] / m["mc_stacks_per_s_matched"]
+    log(f"bench: surrogate inference {out['inference_stacks_per_s']:.3g} stacks/s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +173,7 @@ def main() -> int:
     ap.add_argument("--no-gate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stack workload")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -192,6 +234,8 @@ def main() -> int:
     }
     if not args.no_batched:
         out["batched"] = bench_batched(engine, steps=2, warmup=1)
+    if not args.no_surrogate:
+        out["surrogate"] = bench_surrogate()
     if not args.no_cpu:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mci, photons, batches)

@@ -218,6 +218,8 @@ int tmc_mlp_predict(tmc_mlp *model, const float *x, int64_t n, float *mean, floa
                     float *per_head, double *seconds);
 int tmc_mlp_get_params(tmc_mlp *model, float *params /* [heads x param_count] */);
 int tmc_mlp_set_params(tmc_mlp *model, const float *params);
+/* Learning rate of the following steps (kept in device memory; the captured step reads it). */
+int tmc_mlp_set_lr(tmc_mlp *model, float lr);
 /* Algorithmic matrix-product FLOPs of one training step and of one query (all heads). */
 int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query);
 

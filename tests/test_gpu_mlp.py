@@ -11,7 +11,7 @@ from oracle import mlp_oracle as M
 
 pytestmark = pytest.mark.gpu
 
-CFG = dict(heads=2, inputs=30, hidden=128, depth=3, outputs=202, batch=128, seed=7)
+CFG = dict(heads=2, inputs=30, hidden=128, depth=3, outputs=202, batch=256, seed=7)
 DIMS = (30, 128, 3, 202)
 
 
@@ -34,8 +34,8 @@ def data(n=512, seed=3):
 def test_param_count_and_flops(model):
     assert model.param_count == M.param_count(*DIMS)
     per_step, per_query = model.flops()
-    # padded shapes: in 30 -> 64, out 202 -> 256
-    q = 2 * (128 * 64 + 128 * 128 * 2 + 256 * 128) * 2
+    # padded shapes: in 30 -> 128, out 202 -> 256
+    q = 2 * (128 * 128 + 128 * 128 * 2 + 256 * 128) * 2
     assert per_query == q
 
 
@@ -60,7 +60,7 @@ def test_gradients_match_oracle(model):
     x, y = data(512)
     w = np.linspace(0.5, 2.0, 202).astype(np.float32)
     model.set_data(x, y, w)
-    rows = np.random.default_rng(5).integers(0, 512, 128)
+    rows = np.random.default_rng(5).integers(0, 512, 256)
     g, loss = model.gradients(rows)
     params = model.get_params()
     for h in range(2):
@@ -86,7 +86,7 @@ def test_adam_steps_follow_closed_form(model):
     recursion with the gradient at the new point."""
     x, y = data(1)
     model.set_data(np.repeat(x, 64, 0), np.repeat(y, 64, 0))
-    rows = np.zeros(128, np.int64)
+    rows = np.zeros(256, np.int64)
     p0 = model.get_params().astype(np.float64)
     g0, _ = model.gradients(rows)
     model.train(1)

@@ -406,6 +406,14 @@ int tmc_mlp_set_params(tmc_mlp *model, const float *params) {
     });
 }
 
+int tmc_mlp_set_lr(tmc_mlp *model, float lr) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!(lr >= 0.0f)) config_fail("the learning rate must be >= 0");
+        tmc::mlp_set_lr(*model->impl, lr);
+    });
+}
+
 int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query) {
     return guarded([&] {
         TMC_NEED_MODEL();

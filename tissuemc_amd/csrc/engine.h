@@ -51,6 +51,7 @@ void mlp_gradients(MlpModel &m, const int64_t *rows, float *grads, float *loss);
 double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *spread, float *per_head);
 void mlp_get_params(MlpModel &m, float *out);
 void mlp_set_params(MlpModel &m, const float *in);
+void mlp_set_lr(MlpModel &m, float lr);
 long long mlp_param_count(const MlpModel &m);
 uint64_t mlp_flops_per_step(const MlpModel &m);
 uint64_t mlp_flops_per_query(const MlpModel &m);

@@ -11,12 +11,14 @@
 // so the forward epilogue writes activations in both orientations (H and Ht), the backward
 // epilogue writes dZ and dZt, and Adam keeps bf16 weights in both orientations (Wt and W).
 // Heads are independent networks batched along gridDim.z. Matrices are padded to multiples of
-// 64 with zeros that stay zero (zero inputs, zero loss weights).
+// 128 with zeros that stay zero (zero inputs, zero loss weights). Weight gradients reduce over the
+// batch (K = batch), so they are split along K across workgroups and the partials summed in a
+// fixed order by a second kernel: training is bit-reproducible.
 //
 // MFMA: v_mfma_f32_16x16x32_bf16. Lane l holds A row l%16 and B column l%16, k = 8*(l/16)..+7;
 // the 16x16 result has column l%16, rows 4*(l/16)..+3 (checked on the device). A workgroup of
-// four waves computes a 64x64 tile, each wave 32x32 (2x2 MFMA tiles), K in steps of 32, operands
-// loaded straight from global memory as 16-byte vectors.
+// four waves computes a 128x128 tile, each wave 64x64 (4x4 MFMA tiles), K in steps of 32, operands
+// loaded straight from global memory as 16-byte vectors (see gemm_nt).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,11 +75,13 @@ struct GemmArgs {
     const bf16 *B;
     int M, N, K, lda, ldb;
     long long sA, sB;  // head strides (elements)
+    int splits;        // split-K factor (kBwdWeight only); blockIdx.z = head * splits + split
     // epilogue operands (head strides in elements)
     const float *bias;
     long long sBias;
-    float *Cf;  // fp32 output [M x N] (Z for kFwdHidden, Y for kFwdOut, dW for kBwdWeight)
+    float *Cf;  // fp32 output [M x N] (Z for kFwdHidden, Y for kFwdOut, dW partials for kBwdWeight)
     long long sCf;
+    long long sSplit;  // kBwdWeight: stride between split partials
     bf16 *Cb;   // bf16 output [M x N] (H or dZ)
     bf16 *Ct;   // bf16 transposed output [N x M] (Ht or dZt)
     long long sCb;  // head stride of Cb and Ct (M * N)
@@ -85,47 +89,71 @@ struct GemmArgs {
     long long sZin;
 };
 
+// C[m, n] = sum_k A[m, k] B[n, k] on a 128 x 128 tile per workgroup: four waves in a 2 x 2
+// arrangement, each accumulating 64 x 64 as 4 x 4 MFMA tiles of 16 x 16. Each K step of 32 needs
+// four A and four B fragments per lane (16-byte loads); the next step's fragments are loaded
+// before the current step's 16 MFMAs are issued.
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
-    const int h = static_cast<int>(blockIdx.z);
-    const int m0 = static_cast<int>(blockIdx.y) * 64 + (wave >> 1) * 32;
-    const int n0 = static_cast<int>(blockIdx.x) * 64 + (wave & 1) * 32;
+    const int z = static_cast<int>(blockIdx.z);
+    const int h = z / g.splits, split = z % g.splits;
+    const int m0 = static_cast<int>(blockIdx.y) * 128 + (wave >> 1) * 64;
+    const int n0 = static_cast<int>(blockIdx.x) * 128 + (wave & 1) * 64;
+    const int kspan = g.K / g.splits;
+    const int kbeg = split * kspan, kend = kbeg + kspan;
     const bf16 *A = g.A + h * g.sA;
     const bf16 *B = g.B + h * g.sB;
     const int r = lane & 15, kq = (lane >> 4) * 8;
 
-    floatx4 acc[2][2];
+    floatx4 acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    const bf16 *a0 = A + static_cast<long long>(m0 + r) * g.lda + kq;
-    const bf16 *a1 = a0 + 16LL * g.lda;
-    const bf16 *b0 = B + static_cast<long long>(n0 + r) * g.ldb + kq;
-    const bf16 *b1 = b0 + 16LL * g.ldb;
-    for (int k = 0; k < g.K; k += 32) {
-        const bf16x8 fa0 = *reinterpret_cast<const bf16x8 *>(a0 + k);
-        const bf16x8 fa1 = *reinterpret_cast<const bf16x8 *>(a1 + k);
-        const bf16x8 fb0 = *reinterpret_cast<const bf16x8 *>(b0 + k);
-        const bf16x8 fb1 = *reinterpret_cast<const bf16x8 *>(b1 + k);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+    const bf16 *ap[4], *bp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        ap[i] = A + static_cast<long long>(m0 + 16 * i + r) * g.lda + kq;
+        bp[i] = B + static_cast<long long>(n0 + 16 * i + r) * g.ldb + kq;
+    }
+    bf16x8 fa[4], fb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        fa[i] = *reinterpret_cast<const bf16x8 *>(ap[i] + kbeg);
+        fb[i] = *reinterpret_cast<const bf16x8 *>(bp[i] + kbeg);
+    }
+    for (int k = kbeg; k < kend; k += 32) {
+        bf16x8 na[4], nb[4];
+        const int kn = k + 32 < kend ? k + 32 : k;  // last step reloads the current (unused)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            na[i] = *reinterpret_cast<const bf16x8 *>(ap[i] + kn);
+            nb[i] = *reinterpret_cast<const bf16x8 *>(bp[i] + kn);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            fa[i] = na[i];
+            fb[i] = nb[i];
+        }
     }
 
-    // Epilogue: lane holds rows mbase..mbase+3 of column n of each 16x16 tile.
+    // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             const int n = n0 + 16 * j + r;
             const int mb = m0 + 16 * i + (lane >> 4) * 4;
             if (kEpi == kBwdWeight) {
-                float *C = g.Cf + h * g.sCf;
+                float *C = g.Cf + h * g.sCf + split * g.sSplit;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
             } else if (kEpi == kFwdOut) {
@@ -138,19 +166,19 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
                 bf16 *Ct = g.Ct ? g.Ct + h * g.sCb : nullptr;
                 bf16 out[4];
                 if (kEpi == kFwdHidden) {
-                    float *Z = g.Cf ? g.Cf + h * g.sCf : nullptr;
+                    float *Zp = g.Cf ? g.Cf + h * g.sCf : nullptr;
                     const float b = g.bias[h * g.sBias + n];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const float z = acc[i][j][q] + b;
-                        if (Z) Z[static_cast<long long>(mb + q) * g.N + n] = z;
-                        out[q] = to_bf16(gelu(z));
+                        const float zv = acc[i][j][q] + b;
+                        if (Zp) Zp[static_cast<long long>(mb + q) * g.N + n] = zv;
+                        out[q] = to_bf16(gelu(zv));
                     }
                 } else {  // kBwdData
-                    const float *Z = g.Zin + h * g.sZin;
+                    const float *Zp = g.Zin + h * g.sZin;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        out[q] = to_bf16(acc[i][j][q] * gelu_grad(Z[static_cast<long long>(mb + q) * g.N + n]));
+                        out[q] = to_bf16(acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]));
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) Cb[static_cast<long long>(mb + q) * g.N + n] = out[q];
@@ -162,6 +190,19 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
                 }
             }
         }
+    }
+}
+
+// Sum of the split-K partials of a weight gradient: g[h][e] = sum_s part[h][s][e] for the
+// elements [off, off + count) of each head's block (fixed order: deterministic).
+__global__ void reduce_splits(const float *part, int splits, long long count, long long s_part_head,
+                              long long s_split, float *g, long long s_g_head) {
+    const int h = static_cast<int>(blockIdx.y);
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < count;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        float s = 0.0f;
+        for (int k = 0; k < splits; ++k) s += part[h * s_part_head + k * s_split + e];
+        g[h * s_g_head + e] = s;
     }
 }
 
@@ -247,7 +288,8 @@ struct AdamArgs {
     long long P, Pb;
     int n_mats;
     MatInfo mats[kMaxMats];
-    float lr, beta1, beta2, eps, wd;
+    float beta1, beta2, eps, wd;
+    const float *lr;  // device memory, so a schedule can change it between graph replays
     const uint64_t *step;
 };
 
@@ -265,7 +307,7 @@ __global__ void adam_update(const AdamArgs a) {
         const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
         m[e] = mn;
         v[e] = vn;
-        const float pn = p[e] - a.lr * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
+        const float pn = p[e] - a.lr[0] * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
         p[e] = pn;
         for (int k = 0; k < a.n_mats; ++k) {
             const MatInfo &mi = a.mats[k];
@@ -353,14 +395,14 @@ using mlp::bf16;
 
 class MlpModel {
   public:
-    static int pad64(int x) { return (x + 63) / 64 * 64; }
+    static int pad128(int x) { return (x + 127) / 128 * 128; }
 
     explicit MlpModel(const tmc_mlp_config &c) : cfg_(c) {
         if (c.heads < 1 || c.heads > 64) throw EngineError(TMC_ERR_CONFIG, "heads must be in [1, 64]");
         if (c.depth < 1 || c.depth > mlp::kMaxMats - 1)
             throw EngineError(TMC_ERR_CONFIG, "depth must be in [1, 7]");
-        if (c.hidden < 64 || c.hidden % 64) throw EngineError(TMC_ERR_CONFIG, "hidden must be a multiple of 64");
-        if (c.batch < 64 || c.batch % 64) throw EngineError(TMC_ERR_CONFIG, "batch must be a multiple of 64");
+        if (c.hidden < 128 || c.hidden % 128) throw EngineError(TMC_ERR_CONFIG, "hidden must be a multiple of 128");
+        if (c.batch < 128 || c.batch % 128) throw EngineError(TMC_ERR_CONFIG, "batch must be a multiple of 128");
         if (c.inputs < 1 || c.outputs < 1) throw EngineError(TMC_ERR_CONFIG, "inputs and outputs must be >= 1");
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
@@ -368,8 +410,8 @@ class MlpModel {
         if (c.device < 0 || c.device >= count) throw EngineError(TMC_ERR_CONFIG, "no such device");
         MLP_TRY(hipSetDevice(c.device));
         H_ = c.heads;
-        in_pad_ = pad64(c.inputs);
-        out_pad_ = pad64(c.outputs);
+        in_pad_ = pad128(c.inputs);
+        out_pad_ = pad128(c.outputs);
         hid_ = c.hidden;
         B_ = c.batch;
         Bq_ = std::max(B_, 16384);
@@ -398,10 +440,13 @@ class MlpModel {
         MLP_TRY(hipEventCreate(&e1_));
         p_ = alloc<float>(H_ * P_);
         g_ = alloc<float>(H_ * P_);
+        splits_ = std::max(1, B_ / 256);
+        part_ = alloc<float>(static_cast<long long>(H_) * splits_ * P_);
         m_ = alloc<float>(H_ * P_);
         v_ = alloc<float>(H_ * P_);
         wb_ = alloc<bf16>(H_ * Pb_);
         step_ = alloc<uint64_t>(1);
+        lr_ = alloc<float>(1);
         lossw_ = alloc<float>(out_pad_);
         loss_ = alloc<float>(kLossRing * H_);
         const long long BH = static_cast<long long>(B_) * hid_;
@@ -432,6 +477,7 @@ class MlpModel {
         for (int j = 0; j < c.outputs; ++j) w[j] = 1.0f;
         MLP_TRY(hipMemcpy(lossw_, w.data(), w.size() * 4, hipMemcpyHostToDevice));
         MLP_TRY(hipMemset(step_, 0, 8));
+        set_lr(cfg_.lr);
         MLP_TRY(hipMemset(loss_, 0, static_cast<size_t>(kLossRing) * H_ * 4));
     }
 
@@ -577,6 +623,7 @@ class MlpModel {
         refresh_bf16();
     }
 
+    void set_lr(float lr) { MLP_TRY(hipMemcpy(lr_, &lr, 4, hipMemcpyHostToDevice)); }
     int heads() const { return H_; }
     uint64_t flops_per_step() const {
         uint64_t f = 0;
@@ -660,10 +707,37 @@ class MlpModel {
         MLP_TRY(hipStreamSynchronize(stream_));
     }
 
-    template <int kEpi> void gemm(const mlp::GemmArgs &g) {
-        if (g.M % 64 || g.N % 64 || g.K % 32)
+    template <int kEpi> void gemm(mlp::GemmArgs g) {
+        if (g.splits < 1) g.splits = 1;
+        if (g.M % 128 || g.N % 128 || g.K % (32 * g.splits))
             throw EngineError(TMC_ERR_RUNTIME, "gemm_nt shape is not a multiple of the tile");
-        hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 64, g.M / 64, H_), dim3(256), 0, stream_, g);
+        hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256), 0,
+                           stream_, g);
+        MLP_TRY(hipGetLastError());
+    }
+
+    // dW [out x in] of matrix mi = A (dZt [out x B]) . B (Ht or Xt [in x B])^T, split along the
+    // batch; partials reduced into the gradient block.
+    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht) {
+        mlp::GemmArgs g = base_args();
+        g.A = dzt;
+        g.lda = B_;
+        g.sA = static_cast<long long>(mi.out) * B_;
+        g.B = ht;
+        g.ldb = B_;
+        g.sB = s_ht;
+        g.M = mi.out;
+        g.N = mi.in;
+        g.K = B_;
+        g.splits = splits_;
+        g.Cf = part_ + mi.off;
+        g.sCf = static_cast<long long>(splits_) * P_;
+        g.sSplit = P_;
+        gemm<mlp::kBwdWeight>(g);
+        const long long count = static_cast<long long>(mi.out) * mi.in;
+        hipLaunchKernelGGL(mlp::reduce_splits, dim3(static_cast<unsigned>(std::min<long long>((count + 255) / 256, 1024)), H_),
+                           dim3(256), 0, stream_, part_ + mi.off, splits_, count,
+                           static_cast<long long>(splits_) * P_, P_, g_ + mi.off, P_);
         MLP_TRY(hipGetLastError());
     }
 
@@ -739,19 +813,7 @@ class MlpModel {
         // output layer
         {
             const auto &mo = mats_[D];
-            mlp::GemmArgs g = base_args();
-            g.A = dYt_;
-            g.lda = B_;
-            g.sA = static_cast<long long>(out_pad_) * B_;
-            g.B = Ht_[D - 1];
-            g.ldb = B_;
-            g.sB = BH;
-            g.M = out_pad_;
-            g.N = hid_;
-            g.K = B_;
-            g.Cf = g_ + mo.off;
-            g.sCf = P_;
-            gemm<mlp::kBwdWeight>(g);
+            weight_grad(mo, dYt_, Ht_[D - 1], BH);
             hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_, H_), dim3(256), 0, stream_, dYt_, B_,
                                out_pad_, g_ + mo.boff, P_);
             MLP_TRY(hipGetLastError());
@@ -775,19 +837,8 @@ class MlpModel {
         int cur = 0;
         for (int l = D - 1; l >= 0; --l) {
             const auto &mi = mats_[l];
-            mlp::GemmArgs g = base_args();
-            g.A = dZt_[cur];
-            g.lda = B_;
-            g.sA = BH;
-            g.B = l == 0 ? Xt_ : Ht_[l - 1];
-            g.ldb = B_;
-            g.sB = l == 0 ? static_cast<long long>(B_) * in_pad_ : BH;
-            g.M = hid_;
-            g.N = mi.in;
-            g.K = B_;
-            g.Cf = g_ + mi.off;
-            g.sCf = P_;
-            gemm<mlp::kBwdWeight>(g);
+            weight_grad(mi, dZt_[cur], l == 0 ? Xt_ : Ht_[l - 1],
+                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
             hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
                                g_ + mi.boff, P_);
             MLP_TRY(hipGetLastError());
@@ -823,7 +874,7 @@ class MlpModel {
             a.Pb = Pb_;
             a.n_mats = static_cast<int>(mats_.size());
             for (size_t k = 0; k < mats_.size(); ++k) a.mats[k] = mats_[k];
-            a.lr = cfg_.lr;
+            a.lr = lr_;
             a.beta1 = cfg_.beta1;
             a.beta2 = cfg_.beta2;
             a.eps = cfg_.eps;
@@ -837,14 +888,17 @@ class MlpModel {
     }
 
     tmc_mlp_config cfg_;
-    int H_ = 1, in_pad_ = 64, out_pad_ = 64, hid_ = 64, B_ = 64, Bq_ = 64;
+    int H_ = 1, in_pad_ = 128, out_pad_ = 128, hid_ = 128, B_ = 128, Bq_ = 128;
     long long P_ = 0, Pb_ = 0;
     std::vector<mlp::MatInfo> mats_;
     std::vector<int> real_in_, real_out_;
     std::vector<void *> allocs_;
+    int splits_ = 1;
+    float *part_ = nullptr;
     float *p_ = nullptr, *g_ = nullptr, *m_ = nullptr, *v_ = nullptr, *lossw_ = nullptr, *loss_ = nullptr;
     bf16 *wb_ = nullptr;
     uint64_t *step_ = nullptr;
+    float *lr_ = nullptr;
     bf16 *X_ = nullptr, *Xt_ = nullptr, *dY_ = nullptr, *dYt_ = nullptr;
     float *Yt_ = nullptr, *Yp_ = nullptr;
     std::vector<float *> Z_;
@@ -880,6 +934,7 @@ double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *s
 }
 void mlp_get_params(MlpModel &m, float *out) { m.get_params(out); }
 void mlp_set_params(MlpModel &m, const float *in) { m.set_params(in); }
+void mlp_set_lr(MlpModel &m, float lr) { m.set_lr(lr); }
 long long mlp_param_count(const MlpModel &m) { return m.logical_params(); }
 uint64_t mlp_flops_per_step(const MlpModel &m) { return m.flops_per_step(); }
 uint64_t mlp_flops_per_query(const MlpModel &m) { return m.flops_per_query(); }

@@ -96,7 +96,8 @@ EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_poi
             "tmc_batched_destroy", "tmc_hip_run_batched", "tmc_default_stack_dist",
             "tmc_generate_stacks", "tmc_mlp_default_config", "tmc_mlp_create", "tmc_mlp_destroy",
             "tmc_mlp_param_count", "tmc_mlp_set_data", "tmc_mlp_train", "tmc_mlp_gradients",
-            "tmc_mlp_predict", "tmc_mlp_get_params", "tmc_mlp_set_params", "tmc_mlp_flops")
+            "tmc_mlp_predict", "tmc_mlp_get_params", "tmc_mlp_set_params", "tmc_mlp_set_lr",
+            "tmc_mlp_flops")
 
 _lib: C.CDLL | None = None
 

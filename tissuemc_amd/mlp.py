@@ -48,6 +48,8 @@ def _lib():
         L.tmc_mlp_get_params.restype = C.c_int
         L.tmc_mlp_set_params.argtypes = [C.c_void_p, _PF]
         L.tmc_mlp_set_params.restype = C.c_int
+        L.tmc_mlp_set_lr.argtypes = [C.c_void_p, C.c_float]
+        L.tmc_mlp_set_lr.restype = C.c_int
         L.tmc_mlp_flops.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.tmc_mlp_flops.restype = C.c_int
         _bound = True
@@ -125,6 +127,9 @@ class Mlp:
         p = np.ascontiguousarray(p, np.float32)
         assert p.shape == (self.heads, self.param_count)
         self._check(_lib().tmc_mlp_set_params(self._h, _f(p)))
+
+    def set_lr(self, lr: float) -> None:
+        self._check(_lib().tmc_mlp_set_lr(self._h, lr))
 
     def flops(self) -> tuple[int, int]:
         a, b = C.c_uint64(0), C.c_uint64(0)
