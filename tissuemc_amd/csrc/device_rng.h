@@ -7,8 +7,8 @@
 // al., "Parallel random numbers: as easy as 1, 2, 3" (SC'11), the same published algorithm the
 // reference implements at philox.hpp:26-57.
 //
-// On gfx950 each round is two v_mad_u64_u32 (32x32->64 products) and two three-input XORs; the
-// key schedule is wave-uniform, so the compiler keeps it in scalar registers.
+// On gfx950 each round is two v_mad_u64_u32 (32x32->64 products) and two v_bitop3_b32
+// (three-input XORs); the key schedule is wave-uniform, so it stays in scalar registers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,14 +25,24 @@ struct Block4 {
     uint32_t v0, v1, v2, v3;
 };
 
+// a ^ b ^ c. On the device this is one v_bitop3_b32 (truth table 0x96), a gfx950 instruction
+// the compiler does not form from two XORs by itself.
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 __host__ __device__ __forceinline__ Block4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                          uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
         const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
         const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
-        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3(static_cast<uint32_t>(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3(static_cast<uint32_t>(p0 >> 32), c3, k1);
         c1 = static_cast<uint32_t>(p1);
         c3 = static_cast<uint32_t>(p0);
         c0 = n0;

@@ -28,6 +28,8 @@
 // * Random words: each lane buffers two Philox blocks (8 words). At the top of every step it
 //   refills if fewer than 5 words are left, so the at-most-4 draws of a step never refill in the
 //   middle of divergent code; the words of the step are rotated into r0..r3 once.
+// * Loop-carried lane state is only ever changed under wave-uniform control flow or by select:
+//   state written inside a divergent branch makes the compiler copy it on every iteration.
 // * Math uses the gfx950 transcendental units directly: v_log_f32 (log2), v_sin_f32/v_cos_f32
 //   (argument in revolutions, so the azimuth 2*pi*xi needs no multiply), v_sqrt, v_rsq, v_rcp.
 //   The file is compiled without automatic FMA contraction and every fused multiply-add is
@@ -70,7 +72,10 @@ enum LayerField {
 };
 static_assert(kLayerFields == 16, "layer record is 16 floats");
 
-constexpr int kCachedFields = 11;  // kMut..kHgE live in registers between layer changes
+// The step reads a layer's record as four 16-byte LDS loads; the table starts 16-byte aligned.
+struct alignas(16) LayerRec {
+    float f[kLayerFields];
+};
 
 // Stack record (floats): this header, then kLayerFields floats per layer.
 enum StackHeader { kHdrW0 = 0, kHdrInvDz, kHdrInvDr, kHdrLayers, kStackHeader };
@@ -100,6 +105,7 @@ struct KernelArgs {
     uint32_t n_units, stack_stride;
     int32_t lmax, abs_stride, nz, nr;
     int32_t slot_words, wave_words;    // u64 words: the slot, and slot + layer table, per wave
+                                       // (the table starts at the even word after the slot)
     int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] u32 at word tile_off
     uint32_t chunk;
     uint32_t step_limit;
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     __syncthreads();
 
     uint64_t *slot = lds + wid * a.wave_words;
-    float *ltab = reinterpret_cast<float *>(slot + a.slot_words);
+    float *ltab = reinterpret_cast<float *>(slot + ((a.slot_words + 1) & ~1));
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
     const int off_az = kSlotAbs + a.lmax;
     const uint64_t seed = a.key[0];
@@ -277,255 +283,273 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     bool alive = false;
     float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1, w = 0, sleft = 0;
     int layer = 0;
-    float P[kCachedFields];
-#pragma unroll
-    for (int k = 0; k < kCachedFields; ++k) P[k] = 0.0f;
     uint32_t id_lo = 0, id_hi = 0, blk = 0, idx = 8;
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
     uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
     uint64_t acc = 0;
 
     while (have) {
-        // ---- relaunch: hand the next ids of the wave's unit to lanes without a photon
-        uint64_t need = __ballot(!alive);
-        while (need != 0 && !draining) {
-            if (wu.cur >= wu.end && !claim_chunk(a, wu.u, lane, wu)) {
-                draining = true;
-                break;
-            }
-            const uint32_t n = static_cast<uint32_t>(__popcll(need));
-            const uint64_t avail = wu.end - wu.cur;
-            const uint32_t m = avail < n ? static_cast<uint32_t>(avail) : n;
-            const uint32_t rank = lane_rank(need);
-            if (!alive && rank < m) {
+        for (;;) {
+            // ---- relaunch: hand the next ids of the wave's unit to lanes without a photon
+            uint64_t need = __ballot(!alive);
+            while (need != 0 && !draining) {
+                if (wu.cur >= wu.end && !claim_chunk(a, wu.u, lane, wu)) {
+                    draining = true;
+                    break;
+                }
+                const uint32_t n = static_cast<uint32_t>(__popcll(need));
+                const uint64_t avail = wu.end - wu.cur;
+                const uint32_t m = avail < n ? static_cast<uint32_t>(avail) : n;
+                const uint32_t rank = lane_rank(need);
+                // launch (physics.hpp:58-66), by select on every lane: see the note on loop state.
+                const bool start = !alive && rank < m;
                 const uint64_t id = wu.cur + rank;
-                id_lo = static_cast<uint32_t>(id);
-                id_hi = static_cast<uint32_t>(id >> 32);
-                const Block4 blk0 = philox4x32_10(id_lo, id_hi, 0u, wu.word3, key0, key1);
-                q4 = blk0.v0, q5 = blk0.v1, q6 = blk0.v2, q7 = blk0.v3;
-                blk = 1;
-                idx = 4;
-                x = y = z = 0.0f;
-                ux = uy = 0.0f;
-                uz = 1.0f;
-                w = wu.w0;
-                sleft = 0.0f;
-                layer = 0;
-#pragma unroll
-                for (int k = 0; k < kCachedFields; ++k) P[k] = ltab[k];
-                nstep = 0;
-                alive = true;
+                const uint32_t lo = static_cast<uint32_t>(id), hi = static_cast<uint32_t>(id >> 32);
+                const Block4 blk0 = philox4x32_10(lo, hi, 0u, wu.word3, key0, key1);
+                id_lo = start ? lo : id_lo;
+                id_hi = start ? hi : id_hi;
+                q4 = start ? blk0.v0 : q4;
+                q5 = start ? blk0.v1 : q5;
+                q6 = start ? blk0.v2 : q6;
+                q7 = start ? blk0.v3 : q7;
+                blk = start ? 1u : blk;
+                idx = start ? 4u : idx;
+                x = start ? 0.0f : x;
+                y = start ? 0.0f : y;
+                z = start ? 0.0f : z;
+                ux = start ? 0.0f : ux;
+                uy = start ? 0.0f : uy;
+                uz = start ? 1.0f : uz;
+                w = start ? wu.w0 : w;
+                sleft = start ? 0.0f : sleft;
+                layer = start ? 0 : layer;
+                nstep = start ? 0u : nstep;
+                acc = start ? 0ull : acc;
+                ev = start ? 0u : ev;
+                bs = start ? 0u : bs;
+                an = start ? 0u : an;
+                alive = alive || start;
+                wu.cur += m;
+                need = __ballot(!alive);
             }
-            wu.cur += m;
-            need = __ballot(!alive);
-        }
 
-        if (draining && __ballot(alive) == 0) {
-            // The unit is done for this wave: publish its slot, then find another unit.
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            flush_slot(a, slot, wu, lane);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            have = acquire_unit(a, lane, scan_start, cursor_done, wu);
-            if (have) load_stack(a, wu, ltab, lane);
-            draining = false;
-            continue;
-        }
+            if (draining && __ballot(alive) == 0) break;
 
-        if (!alive) continue;
+            // ---- one hop-drop-spin step (physics.hpp:264-277). Every lane runs it, so that no
+            // divergent branch encloses the loop-carried state; a lane without a photon (only while
+            // its wave drains a unit) steps stale values and scores nothing. The layer's constants
+            // are read from the wave's LDS table at every step rather than carried in registers.
+            const LayerRec P = reinterpret_cast<const LayerRec *>(ltab)[layer];
+            {
+                // Refill when fewer than 4 buffered words are left. In a wave some lane needs
+                // a block at nearly every step, so every lane computes its next block and keeps
+                // it by select: a branch would cost the same block plus copies of the buffer.
+                const Block4 nb = philox4x32_10(id_lo, id_hi, blk, wu.word3, key0, key1);
+                const bool take = idx >= 4;
+                q0 = take ? q4 : q0;
+                q1 = take ? q5 : q1;
+                q2 = take ? q6 : q2;
+                q3 = take ? q7 : q3;
+                q4 = take ? nb.v0 : q4;
+                q5 = take ? nb.v1 : q5;
+                q6 = take ? nb.v2 : q6;
+                q7 = take ? nb.v3 : q7;
+                blk += take ? 1u : 0u;
+                idx -= take ? 4u : 0u;
+            }
+            // r_j = word idx + j of the 8 buffered words (idx in 0..3)
+            const bool i1 = (idx & 1u) != 0, i2 = (idx & 2u) != 0;
+            const uint32_t r0 = i2 ? (i1 ? q3 : q2) : (i1 ? q1 : q0);
+            const uint32_t r1 = i2 ? (i1 ? q4 : q3) : (i1 ? q2 : q1);
+            const uint32_t r2 = i2 ? (i1 ? q5 : q4) : (i1 ? q3 : q2);
+            const uint32_t r3 = i2 ? (i1 ? q6 : q5) : (i1 ? q4 : q3);
 
+            // step_size (physics.hpp:149-158)
+            const bool fresh = sleft == 0.0f;
+            float s = fresh ? __builtin_amdgcn_logf(unit_open(r0)) * P.f[kNl2InvMut]
+                            : sleft * P.f[kInvMut];
+            uint32_t c = fresh ? 1u : 0u;
+            sleft = 0.0f;
 
-        // ---- one hop-drop-spin step (physics.hpp:264-277)
-        if (idx >= 4) {
-            q0 = q4, q1 = q5, q2 = q6, q3 = q7;
-            const Block4 nb = philox4x32_10(id_lo, id_hi, blk, wu.word3, key0, key1);
-            q4 = nb.v0, q5 = nb.v1, q6 = nb.v2, q7 = nb.v3;
-            ++blk;
-            idx -= 4;
-        }
-        // r_j = word idx + j of the 8 buffered words (idx in 0..3)
-        const bool i1 = (idx & 1u) != 0, i2 = (idx & 2u) != 0;
-        const uint32_t r0 = i2 ? (i1 ? q3 : q2) : (i1 ? q1 : q0);
-        const uint32_t r1 = i2 ? (i1 ? q4 : q3) : (i1 ? q2 : q1);
-        const uint32_t r2 = i2 ? (i1 ? q5 : q4) : (i1 ? q3 : q2);
-        const uint32_t r3 = i2 ? (i1 ? q6 : q5) : (i1 ? q4 : q3);
+            // hit_boundary (physics.hpp:162-178)
+            const float zb = uz > 0.0f ? P.f[kZ1] : P.f[kZ0];
+            const float dl = (zb - z) * __builtin_amdgcn_rcpf(uz);
+            const bool hit = (uz != 0.0f) && (s > dl);
+            if (hit) {
+                sleft = (s - dl) * P.f[kMut];
+                s = dl;
+            }
+            // hop (physics.hpp:180-184)
+            x = __builtin_fmaf(s, ux, x);
+            y = __builtin_fmaf(s, uy, y);
+            z = __builtin_fmaf(s, uz, z);
 
-        // step_size (physics.hpp:149-158)
-        const bool fresh = sleft == 0.0f;
-        float s = fresh ? __builtin_amdgcn_logf(unit_open(r0)) * P[kNl2InvMut] : sleft * P[kInvMut];
-        uint32_t c = fresh ? 1u : 0u;
-        sleft = 0.0f;
-
-        // hit_boundary (physics.hpp:162-178)
-        const float zb = uz > 0.0f ? P[kZ1] : P[kZ0];
-        const float dl = (zb - z) * __builtin_amdgcn_rcpf(uz);
-        const bool hit = (uz != 0.0f) && (s > dl);
-        if (hit) {
-            sleft = (s - dl) * P[kMut];
-            s = dl;
-        }
-        // hop (physics.hpp:180-184)
-        x = __builtin_fmaf(s, ux, x);
-        y = __builtin_fmaf(s, uy, y);
-        z = __builtin_fmaf(s, uz, z);
-
-        bool died = false;
-        if (hit) {
-            // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
-            ++bs;
-            const float *lt = ltab + layer * kLayerFields;
-            const bool up = uz < 0.0f;
-            const float ni = lt[kN];
-            const float nt = up ? lt[kNUp] : lt[kNDown];
-            const float cc = up ? lt[kCcTop] : lt[kCcBot];
-            const float ca1 = up ? -uz : uz;
-            float r = 1.0f, ca2 = 0.0f;
-            if (ca1 > cc) {
-                if (ni == nt) {
-                    r = 0.0f;
-                    ca2 = ca1;
-                } else if (ca1 > 1.0f - 1e-6f) {
-                    const float t = (nt - ni) * __builtin_amdgcn_rcpf(nt + ni);
-                    r = t * t;
-                    ca2 = ca1;
-                } else if (ca1 < 1e-6f) {
-                    r = 1.0f;
-                } else {
-                    const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
-                    const float sa2 = ni * sa1 * __builtin_amdgcn_rcpf(nt);
-                    if (sa2 >= 1.0f) {
+            bool died = false;
+            if (hit) {
+                // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
+                ++bs;
+                const bool up = uz < 0.0f;
+                const float ni = P.f[kN];
+                const float nt = up ? P.f[kNUp] : P.f[kNDown];
+                const float cc = up ? P.f[kCcTop] : P.f[kCcBot];
+                const float ca1 = up ? -uz : uz;
+                float r = 1.0f, ca2 = 0.0f;
+                if (ca1 > cc) {
+                    if (ni == nt) {
+                        r = 0.0f;
+                        ca2 = ca1;
+                    } else if (ca1 > 1.0f - 1e-6f) {
+                        const float t = (nt - ni) * __builtin_amdgcn_rcpf(nt + ni);
+                        r = t * t;
+                        ca2 = ca1;
+                    } else if (ca1 < 1e-6f) {
                         r = 1.0f;
                     } else {
-                        ca2 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-sa2, sa2, 1.0f));
-                        const float cap = __builtin_fmaf(ca1, ca2, -(sa1 * sa2));
-                        const float cam = __builtin_fmaf(ca1, ca2, sa1 * sa2);
-                        const float sap = __builtin_fmaf(sa1, ca2, ca1 * sa2);
-                        const float sam = __builtin_fmaf(sa1, ca2, -(ca1 * sa2));
-                        r = 0.5f * sam * sam * __builtin_fmaf(cam, cam, cap * cap) *
-                            __builtin_amdgcn_rcpf(sap * sap * cam * cam);
+                        const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
+                        const float sa2 = ni * sa1 * __builtin_amdgcn_rcpf(nt);
+                        if (sa2 >= 1.0f) {
+                            r = 1.0f;
+                        } else {
+                            ca2 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-sa2, sa2, 1.0f));
+                            const float cap = __builtin_fmaf(ca1, ca2, -(sa1 * sa2));
+                            const float cam = __builtin_fmaf(ca1, ca2, sa1 * sa2);
+                            const float sap = __builtin_fmaf(sa1, ca2, ca1 * sa2);
+                            const float sam = __builtin_fmaf(sa1, ca2, -(ca1 * sa2));
+                            r = 0.5f * sam * sam * __builtin_fmaf(cam, cam, cap * cap) *
+                                __builtin_amdgcn_rcpf(sap * sap * cam * cam);
+                        }
                     }
                 }
-            }
-            const float xi = unit_open(c ? r1 : r0);
-            ++c;
-            if (xi > r) {
-                const float ratio = ni * __builtin_amdgcn_rcpf(nt);
-                ux *= ratio;
-                uy *= ratio;
-                uz = up ? -ca2 : ca2;
-                const bool exit_top = up && layer == 0;
-                const bool exit_bot = !up && layer == wu.nl - 1;
-                if (exit_top || exit_bot) {
-                    an += bad_score(ux, uy, uz, w);
-                    const uint64_t qw = to_fixed(w);
-                    lds_add(slot + (exit_top ? kSlotRd : kSlotTt), qw);
-                    if (kProfiles) {
+                const float xi = unit_open(c ? r1 : r0);
+                ++c;
+                if (xi > r) {
+                    const float ratio = ni * __builtin_amdgcn_rcpf(nt);
+                    ux *= ratio;
+                    uy *= ratio;
+                    uz = up ? -ca2 : ca2;
+                    const bool exit_top = up && layer == 0;
+                    const bool exit_bot = !up && layer == wu.nl - 1;
+                    if (exit_top || exit_bot) {
+                        an += bad_score(ux, uy, uz, w);
+                        if (alive) {
+                            const uint64_t qw = to_fixed(w);
+                            lds_add(slot + (exit_top ? kSlotRd : kSlotTt), qw);
+                            if (kProfiles) {
+                                const float rr =
+                                    __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
+                                const int ir = bin_of(rr, wu.inv_dr, a.nr);
+                                uint64_t *row = (exit_top ? a.rr : a.tr) +
+                                                static_cast<uint64_t>(wu.prow) * a.nr;
+                                hbm_add(row + ir, qw);
+                            }
+                        }
+                        died = true;
+                    } else {
+                        if (alive && acc != 0) lds_add(slot + kSlotAbs + layer, acc);
+                        acc = 0;
+                        layer += up ? -1 : 1;
+                    }
+                } else {
+                    uz = -uz;
+                }
+            } else {
+                // drop (physics.hpp:188-193) and its score (gpu_tally.hpp:51-56)
+                const float dwa = w * P.f[kAlbedoA];
+                w -= dwa;
+                ++ev;
+                an += bad_score(ux, uy, uz, dwa);
+                const uint64_t q = to_fixed(dwa);
+                acc += q;
+                if (kProfiles && alive) {
+                    const int iz = bin_of(z, wu.inv_dz, a.nz);
+                    lds_add(slot + off_az + iz, q);
+                    if (kArz) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                         const int ir = bin_of(rr, wu.inv_dr, a.nr);
-                        hbm_add((exit_top ? a.rr : a.tr) + static_cast<uint64_t>(wu.prow) * a.nr + ir, qw);
+                        uint64_t *hbm_bin = a.arz + static_cast<uint64_t>(iz) * a.nr + ir;
+                        if (iz < a.tile_z && ir < a.tile_r) {
+                            const uint32_t lo = static_cast<uint32_t>(q);
+                            const uint32_t old = __hip_atomic_fetch_add(
+                                tile + iz * a.tile_r + ir, lo, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                            const uint64_t carry = (q >> 32) + (old + lo < old ? 1u : 0u);
+                            if (carry != 0) hbm_add(hbm_bin, carry << 32);
+                        } else {
+                            hbm_add(hbm_bin, q);
+                        }
                     }
+                }
+                // spin (physics.hpp:116-144) with the HG sampler (physics.hpp:101-113)
+                const float xi_hg = unit_open(c ? r1 : r0);
+                const float xi_az = unit_open(c ? r2 : r1);
+                c += 2;
+                float cost;
+                if (P.f[kHgC] == 0.0f) {
+                    cost = __builtin_fmaf(2.0f, xi_hg, -1.0f);
+                } else {
+                    const float den = __builtin_fmaf(P.f[kHgC], xi_hg, P.f[kHgB]);
+                    const float t = P.f[kHgA] * __builtin_amdgcn_rcpf(den);
+                    cost = __builtin_fmaf(-t, t, P.f[kHgD]) * P.f[kHgE];
+                    cost = __builtin_fminf(__builtin_fmaxf(cost, -1.0f), 1.0f);
+                }
+                const float s2 = __builtin_fmaf(-cost, cost, 1.0f);
+                const float sint = s2 > 0.0f ? __builtin_amdgcn_sqrtf(s2) : 0.0f;
+                const float cosp = __builtin_amdgcn_cosf(xi_az);  // cos(2 pi xi)
+                const float sinp = __builtin_amdgcn_sinf(xi_az);  // sin(2 pi xi)
+                float nx, ny, nz_;
+                if (__builtin_fabsf(uz) > 1.0f - 1e-6f) {
+                    nx = sint * cosp;
+                    ny = sint * sinp;
+                    nz_ = uz >= 0.0f ? cost : -cost;
+                } else {
+                    const float t2 = __builtin_fmaf(-uz, uz, 1.0f);
+                    const float rt = __builtin_amdgcn_rsqf(t2);
+                    const float temp = t2 * rt;
+                    const float sr = sint * rt;
+                    nx = __builtin_fmaf(sr, __builtin_fmaf(ux * uz, cosp, -(uy * sinp)), ux * cost);
+                    ny = __builtin_fmaf(sr, __builtin_fmaf(uy * uz, cosp, ux * sinp), uy * cost);
+                    nz_ = __builtin_fmaf(-(sint * cosp), temp, uz * cost);
+                }
+                const float n2 = __builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz_ * nz_));
+                const float inv = __builtin_amdgcn_rsqf(n2);
+                ux = nx * inv;
+                uy = ny * inv;
+                uz = nz_ * inv;
+            }
+            // roulette (physics.hpp:250-259, 274-276)
+            if (!died && w < 1e-4f) {
+                if (w == 0.0f) {
                     died = true;
                 } else {
-                    if (acc != 0) {
-                        lds_add(slot + kSlotAbs + layer, acc);
-                        acc = 0;
-                    }
-                    layer += up ? -1 : 1;
-                    const float *nl = ltab + layer * kLayerFields;
-#pragma unroll
-                    for (int k = 0; k < kCachedFields; ++k) P[k] = nl[k];
-                }
-            } else {
-                uz = -uz;
-            }
-        } else {
-            // drop (physics.hpp:188-193) and its score (gpu_tally.hpp:51-56)
-            const float dwa = w * P[kAlbedoA];
-            w -= dwa;
-            ++ev;
-            an += bad_score(ux, uy, uz, dwa);
-            const uint64_t q = to_fixed(dwa);
-            acc += q;
-            if (kProfiles) {
-                const int iz = bin_of(z, wu.inv_dz, a.nz);
-                lds_add(slot + off_az + iz, q);
-                if (kArz) {
-                    const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                    const int ir = bin_of(rr, wu.inv_dr, a.nr);
-                    uint64_t *hbm_bin = a.arz + static_cast<uint64_t>(iz) * a.nr + ir;
-                    if (iz < a.tile_z && ir < a.tile_r) {
-                        const uint32_t lo = static_cast<uint32_t>(q);
-                        const uint32_t old = __hip_atomic_fetch_add(
-                            tile + iz * a.tile_r + ir, lo, __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                        const uint64_t carry = (q >> 32) + (old + lo < old ? 1u : 0u);
-                        if (carry != 0) hbm_add(hbm_bin, carry << 32);
-                    } else {
-                        hbm_add(hbm_bin, q);
-                    }
+                    const uint32_t rw = c == 1 ? r1 : (c == 2 ? r2 : r3);
+                    ++c;
+                    if (unit_open(rw) < 0.1f)
+                        w = w / 0.1f;
+                    else
+                        died = true;
                 }
             }
-            // spin (physics.hpp:116-144) with the HG sampler (physics.hpp:101-113)
-            const float xi_hg = unit_open(c ? r1 : r0);
-            const float xi_az = unit_open(c ? r2 : r1);
-            c += 2;
-            float cost;
-            if (P[kHgC] == 0.0f) {
-                cost = __builtin_fmaf(2.0f, xi_hg, -1.0f);
-            } else {
-                const float t = P[kHgA] * __builtin_amdgcn_rcpf(__builtin_fmaf(P[kHgC], xi_hg, P[kHgB]));
-                cost = __builtin_fmaf(-t, t, P[kHgD]) * P[kHgE];
-                cost = __builtin_fminf(__builtin_fmaxf(cost, -1.0f), 1.0f);
-            }
-            const float s2 = __builtin_fmaf(-cost, cost, 1.0f);
-            const float sint = s2 > 0.0f ? __builtin_amdgcn_sqrtf(s2) : 0.0f;
-            const float cosp = __builtin_amdgcn_cosf(xi_az);  // cos(2 pi xi)
-            const float sinp = __builtin_amdgcn_sinf(xi_az);  // sin(2 pi xi)
-            float nx, ny, nz_;
-            if (__builtin_fabsf(uz) > 1.0f - 1e-6f) {
-                nx = sint * cosp;
-                ny = sint * sinp;
-                nz_ = uz >= 0.0f ? cost : -cost;
-            } else {
-                const float t2 = __builtin_fmaf(-uz, uz, 1.0f);
-                const float rt = __builtin_amdgcn_rsqf(t2);
-                const float temp = t2 * rt;
-                const float sr = sint * rt;
-                nx = __builtin_fmaf(sr, __builtin_fmaf(ux * uz, cosp, -(uy * sinp)), ux * cost);
-                ny = __builtin_fmaf(sr, __builtin_fmaf(uy * uz, cosp, ux * sinp), uy * cost);
-                nz_ = __builtin_fmaf(-(sint * cosp), temp, uz * cost);
-            }
-            const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(nx, nx, __builtin_fmaf(ny, ny, nz_ * nz_)));
-            ux = nx * inv;
-            uy = ny * inv;
-            uz = nz_ * inv;
-        }
-        // roulette (physics.hpp:250-259, 274-276)
-        if (!died && w < 1e-4f) {
-            if (w == 0.0f) {
+            idx += c;
+            if (++nstep >= a.step_limit && !died) {
+                ++an;  // a photon this long is reported, not silently dropped
                 died = true;
-            } else {
-                const uint32_t rw = c == 1 ? r1 : (c == 2 ? r2 : r3);
-                ++c;
-                if (unit_open(rw) < 0.1f)
-                    w = w / 0.1f;
-                else
-                    died = true;
             }
+            if (died && alive) {
+                if (acc != 0) lds_add(slot + kSlotAbs + layer, acc);
+                if (ev != 0) lds_add(slot + kSlotEvents, ev);
+                if (bs != 0) lds_add(slot + kSlotBoundary, bs);
+                if (an != 0) lds_add(slot + kSlotAnomalies, an);
+            }
+            alive = alive && !died;
         }
-        idx += c;
-        if (++nstep >= a.step_limit && !died) {
-            ++an;  // a photon this long is reported, not silently dropped
-            died = true;
-        }
-        if (died) {
-            if (acc != 0) lds_add(slot + kSlotAbs + layer, acc);
-            if (ev != 0) lds_add(slot + kSlotEvents, ev);
-            if (bs != 0) lds_add(slot + kSlotBoundary, bs);
-            if (an != 0) lds_add(slot + kSlotAnomalies, an);
-            acc = 0;
-            ev = bs = an = 0;
-            alive = false;
-        }
+        // The unit is done for this wave: publish its slot, then find another unit.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        flush_slot(a, slot, wu, lane);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        have = acquire_unit(a, lane, scan_start, cursor_done, wu);
+        if (have) load_stack(a, wu, ltab, lane);
+        layer = 0;  // keeps the idle lanes' table index inside the new stack
+        draining = false;
     }
 
     if (kArz) {
@@ -678,7 +702,7 @@ class Resident {
         profiles_ = p.profiles;
         arz_ = p.arz;
         a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
-        a_.wave_words = a_.slot_words + (p.lmax * kLayerFields + 1) / 2;
+        a_.wave_words = ((a_.slot_words + 1) & ~1) + p.lmax * kLayerFields / 2;
 
         const size_t cap = dev.lds_per_block;
         int wpg = t.waves_per_group > 0 ? t.waves_per_group : 16;
