@@ -66,16 +66,38 @@ enum LayerField {
     kN,
     kNUp,    // index above the layer
     kNDown,  // index below the layer
-    kCcTop,  // cosine of the critical angle going up (0 if none)
-    kCcBot,  // going down
+    kCcTop,    // cosine of the critical angle going up (0 if none)
+    kCcBot,    // going down
+    kEtaUp,    // n / n_above: Snell's ratio sin(t) / sin(i) going up
+    kEtaDown,  // n / n_below
+    kR0Up,     // Fresnel reflectance at normal incidence going up, ((n - n_above) / (n + n_above))^2
+    kR0Down,   // going down
     kLayerFields
 };
-static_assert(kLayerFields == 16, "layer record is 16 floats");
+static_assert(kLayerFields == 20, "layer record is 20 floats");
 
-// The step reads a layer's record as four 16-byte LDS loads; the table starts 16-byte aligned.
-struct alignas(16) LayerRec {
+// The step reads a layer's record as 16-byte LDS loads; the table starts 16-byte aligned.
+// Pointers into LDS are kept in address space 3, so that their arithmetic stays 32-bit.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const f32x4 lds_f32x4;
+typedef __attribute__((address_space(3))) const float lds_float;
+struct LayerRec {
     float f[kLayerFields];
 };
+__device__ __forceinline__ LayerRec load_layer(uint32_t ltab, int layer) {
+    const uint32_t at = ltab + __umul24(static_cast<uint32_t>(layer), kLayerFields * sizeof(float));
+    const lds_f32x4 *p = (const lds_f32x4 *)(uintptr_t)at;
+    LayerRec r;
+#pragma unroll
+    for (int k = 0; k < kLayerFields / 4; ++k) {
+        const f32x4 v = p[k];
+        r.f[4 * k] = v.x;
+        r.f[4 * k + 1] = v.y;
+        r.f[4 * k + 2] = v.z;
+        r.f[4 * k + 3] = v.w;
+    }
+    return r;
+}
 
 // Stack record (floats): this header, then kLayerFields floats per layer.
 enum StackHeader { kHdrW0 = 0, kHdrInvDz, kHdrInvDr, kHdrLayers, kStackHeader };
@@ -130,10 +152,12 @@ __device__ __forceinline__ uint64_t to_fixed(float w) {
     return y >= 4294967296.0f ? (1ull << 32) : static_cast<uint64_t>(lo);
 }
 
-__device__ __forceinline__ int bin_of(float x, float inv_width, int count) {
-    if (!(x > 0.0f)) return 0;
-    const float i = x * inv_width;
-    return i >= static_cast<float>(count) ? count - 1 : static_cast<int>(i);
+// Bin of x >= 0 with both ends clamped (gpu_tally.hpp:51-56): x <= 0 and NaN give bin 0, x past
+// the grid the last bin. The clamp is done in float (last = count - 1, exact below 2^24), so the
+// conversion only ever sees values in [0, count - 1].
+__device__ __forceinline__ int bin_of(float x, float inv_width, float last) {
+    const float i = __builtin_fminf(__builtin_fmaxf(x * inv_width, 0.0f), last);
+    return static_cast<int>(i);
 }
 
 __device__ __forceinline__ void lds_add(uint64_t *p, uint64_t v) {
@@ -263,8 +287,10 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
 
     uint64_t *slot = lds + wid * a.wave_words;
     float *ltab = reinterpret_cast<float *>(slot + ((a.slot_words + 1) & ~1));
+    const uint32_t ltab_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_float *)ltab));
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
     const int off_az = kSlotAbs + a.lmax;
+    const float last_z = static_cast<float>(a.nz - 1), last_r = static_cast<float>(a.nr - 1);
     const uint64_t seed = a.key[0];
     const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
 
@@ -339,7 +365,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             // divergent branch encloses the loop-carried state; a lane without a photon (only while
             // its wave drains a unit) steps stale values and scores nothing. The layer's constants
             // are read from the wave's LDS table at every step rather than carried in registers.
-            const LayerRec P = reinterpret_cast<const LayerRec *>(ltab)[layer];
+            const LayerRec P = load_layer(ltab_lds, layer);
             {
                 // Refill when fewer than 4 buffered words are left. In a wave some lane needs
                 // a block at nearly every step, so every lane computes its next block and keeps
@@ -392,6 +418,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 const float ni = P.f[kN];
                 const float nt = up ? P.f[kNUp] : P.f[kNDown];
                 const float cc = up ? P.f[kCcTop] : P.f[kCcBot];
+                const float eta = up ? P.f[kEtaUp] : P.f[kEtaDown];
                 const float ca1 = up ? -uz : uz;
                 float r = 1.0f, ca2 = 0.0f;
                 if (ca1 > cc) {
@@ -399,14 +426,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                         r = 0.0f;
                         ca2 = ca1;
                     } else if (ca1 > 1.0f - 1e-6f) {
-                        const float t = (nt - ni) * __builtin_amdgcn_rcpf(nt + ni);
-                        r = t * t;
+                        r = up ? P.f[kR0Up] : P.f[kR0Down];
                         ca2 = ca1;
                     } else if (ca1 < 1e-6f) {
                         r = 1.0f;
                     } else {
                         const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
-                        const float sa2 = ni * sa1 * __builtin_amdgcn_rcpf(nt);
+                        const float sa2 = sa1 * eta;
                         if (sa2 >= 1.0f) {
                             r = 1.0f;
                         } else {
@@ -423,9 +449,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 const float xi = unit_open(c ? r1 : r0);
                 ++c;
                 if (xi > r) {
-                    const float ratio = ni * __builtin_amdgcn_rcpf(nt);
-                    ux *= ratio;
-                    uy *= ratio;
+                    ux *= eta;
+                    uy *= eta;
                     uz = up ? -ca2 : ca2;
                     const bool exit_top = up && layer == 0;
                     const bool exit_bot = !up && layer == wu.nl - 1;
@@ -437,7 +462,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                             if (kProfiles) {
                                 const float rr =
                                     __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                                const int ir = bin_of(rr, wu.inv_dr, a.nr);
+                                const int ir = bin_of(rr, wu.inv_dr, last_r);
                                 uint64_t *row = (exit_top ? a.rr : a.tr) +
                                                 static_cast<uint64_t>(wu.prow) * a.nr;
                                 hbm_add(row + ir, qw);
@@ -460,22 +485,28 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 an += bad_score(ux, uy, uz, dwa);
                 const uint64_t q = to_fixed(dwa);
                 acc += q;
+                // A(r,z) tile add: its carry is resolved after the spin, so that the LDS atomic's
+                // returned value is not waited for right away.
+                bool tile_pending = false;
+                uint32_t tile_old = 0, tile_bin = 0;
                 if (kProfiles && alive) {
-                    const int iz = bin_of(z, wu.inv_dz, a.nz);
+                    const int iz = bin_of(z, wu.inv_dz, last_z);
                     lds_add(slot + off_az + iz, q);
                     if (kArz) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                        const int ir = bin_of(rr, wu.inv_dr, a.nr);
-                        uint64_t *hbm_bin = a.arz + static_cast<uint64_t>(iz) * a.nr + ir;
+                        const int ir = bin_of(rr, wu.inv_dr, last_r);
+                        tile_bin = __umul24(static_cast<uint32_t>(iz), static_cast<uint32_t>(a.nr)) +
+                                   static_cast<uint32_t>(ir);
                         if (iz < a.tile_z && ir < a.tile_r) {
-                            const uint32_t lo = static_cast<uint32_t>(q);
-                            const uint32_t old = __hip_atomic_fetch_add(
-                                tile + iz * a.tile_r + ir, lo, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                            const uint64_t carry = (q >> 32) + (old + lo < old ? 1u : 0u);
-                            if (carry != 0) hbm_add(hbm_bin, carry << 32);
+                            const uint32_t t = __umul24(static_cast<uint32_t>(iz),
+                                                        static_cast<uint32_t>(a.tile_r)) +
+                                               static_cast<uint32_t>(ir);
+                            tile_old = __hip_atomic_fetch_add(tile + t, static_cast<uint32_t>(q),
+                                                              __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                            tile_pending = true;
                         } else {
-                            hbm_add(hbm_bin, q);
+                            hbm_add(a.arz + tile_bin, q);
                         }
                     }
                 }
@@ -515,6 +546,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 ux = nx * inv;
                 uy = ny * inv;
                 uz = nz_ * inv;
+                if (kArz && tile_pending) {
+                    // A counter that wrapped carries 2^32 into the HBM bin (exact: the atomic
+                    // returned the value it added to).
+                    const uint32_t lo = static_cast<uint32_t>(q);
+                    const uint64_t carry = (q >> 32) + (tile_old + lo < tile_old ? 1u : 0u);
+                    if (carry != 0) hbm_add(a.arz + tile_bin, carry << 32);
+                }
             }
             // roulette (physics.hpp:250-259, 274-276)
             if (!died && w < 1e-4f) {
@@ -636,6 +674,11 @@ void fill_stack(int nl, double n_above, double n_below, const tmc_layer *layers,
         r[kNDown] = static_cast<float>(n_bot);
         r[kCcTop] = static_cast<float>(critical_cos(L.n, n_top));
         r[kCcBot] = static_cast<float>(critical_cos(L.n, n_bot));
+        r[kEtaUp] = static_cast<float>(L.n / n_top);
+        r[kEtaDown] = static_cast<float>(L.n / n_bot);
+        const double t_up = (n_top - L.n) / (n_top + L.n), t_down = (n_bot - L.n) / (n_bot + L.n);
+        r[kR0Up] = static_cast<float>(t_up * t_up);
+        r[kR0Down] = static_cast<float>(t_down * t_down);
     }
 }
 
