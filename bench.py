@@ -1,11 +1,13 @@
 """Throughput of the MI355X photon-transport engine on the three-layer workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--no-arz]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--no-arz] [--no-tune]
 
 One step is one complete run of the workload (three-layer stack, 1e8 photons, 200 x 200 grid of
 0.005 cm bins, 100 batches, A_z/R_r/T_r/A(r,z) tallies) through the C ABI. Before timing, the same
 kernel is checked photon for photon against the oracle (the reference's identical-photon limits);
-a failing check aborts the benchmark. Prints one JSON line.
+a failing check aborts the benchmark. The launch shape is then picked by the engine's autotuner
+from pilot runs at 1/32 of the photons (outside the timed region; tallies do not depend on the
+shape). Prints one JSON line.
 
 roofline: the transport kernel is bound by vector ALU issue (not HBM, not the matrix cores).
 Its algorithmic work is 190 VALU lane-op slots per absorption event and 100 per boundary step
@@ -96,13 +98,25 @@ def cpu_baseline(mci: Path, full_photons: int, batches: int, target_s: float = 1
                       f"{t:.1f} s wall, {min(cores, batches)} threads"}
 
 
-def bench_batched(engine, steps: int, warmup: int) -> dict:
+def tune_summary(rep: dict) -> dict:
+    best = rep["tried"][rep["best"]]
+    return {"pilot_fraction": rep["pilot_fraction"], "chosen": best,
+            "tried": [[t["waves_per_group"], t["groups"], round(t["kernel_seconds"] * 1e3, 3)]
+                      for t in rep["tried"]]}
+
+
+def bench_batched(engine, steps: int, warmup: int, tune: bool) -> dict:
     """The batched workload: 4096 stacks (1-5 layers, budgets log-uniform 1e4-1e6), 10 batches per
-    stack, 100 x 100 grid at 0.01 cm, one graph-replayed persistent launch per step."""
+    stack, 100 x 100 grid at 0.01 cm, one graph-replayed persistent launch per step, at the
+    launch shape the autotuner picks from pilot runs (1/16 of every budget)."""
     stacks = engine.generate_stacks(2026, 0, 4096)
     photons = sum(int(stacks[i].photons) for i in range(4096))
+    tuned = None
     with engine.BatchedPlan(4096, batches_per_stack=10, dz=0.01, dr=0.01, nz=100, nr=100,
                             photons_hint=photons) as plan:
+        if tune:
+            tuned = tune_summary(plan.autotune(stacks, seed=1, pilot_fraction=1 / 16))
+            log(f"bench: batched autotune chose {tuned['chosen']}")
         for _ in range(warmup):
             plan.execute(stacks, seed=1)
         t0 = time.perf_counter()
@@ -119,10 +133,10 @@ def bench_batched(engine, steps: int, warmup: int) -> dict:
             "kernel_ms_per_step": kernel / steps * 1e3,
             "roofline_frac": (events * SLOTS_PER_EVENT + bsteps * SLOTS_PER_BOUNDARY_STEP)
                              / kernel / PEAK_LANE_OPS,
-            "launch": res[0]["launch"]}
+            "launch": res[0]["launch"], "autotune": tuned}
 
 
-def bench_surrogate() -> dict:
+def bench_surrogate(mc_photons_per_s: float | None = None) -> dict:
     """The surrogate's kernels at full size: ensemble inference on 1.05e6 queries and a burst
     of training steps (4 heads, 30 -> 4 x 512 GELU -> 202, batch 4096), device time by HIP
     events; errors and the matched-error comparison come from the full pipeline run recorded in
@@ -157,8 +171,15 @@ def bench_surrogate() -> dict:
             "routed_frac": r["inference"]["routed_frac"],
             "mc_stacks_per_s_at_matched_err": m["mc_stacks_per_s_matched"],
             "speedup_in_domain_recorded": m["speedup_in_domain"],
-            "speedup_with_fallback_recorded": m["speedup_with_fallback"]}
+            "speedup_with_fallback_recorded": m["speedup_with_fallback"],
+            "mc_photons_for_matched_err": m["mc_photons_for_matched_err"]}
         out["speedup_in_domain_live"] = out["inference_stacks_per_s"] / m["mc_stacks_per_s_matched"]
+        if mc_photons_per_s:
+            # batched MC at this run's measured rate, with the photons per stack it needs to match
+            # the surrogate's recorded median R_d error
+            mc_live = mc_photons_per_s / m["mc_photons_for_matched_err"]
+            out["mc_stacks_per_s_matched_live"] = mc_live
+            out["speedup_in_domain_live"] = out["inference_stacks_per_s"] / mc_live
     log(f"bench: surrogate inference {out['inference_stacks_per_s']:.3g} stacks/s")
     return out
 
@@ -174,6 +195,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stack workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate kernels")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default launch shape instead of autotuning it from pilot runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -189,13 +212,20 @@ def main() -> int:
     if not args.no_gate:
         gate(engine, spec)
 
+    shape, tuned = {}, None
+    if not args.no_tune:
+        t = engine.autotune(spec, photons=photons, batches=batches, seed=1, arz=arz,
+                            pilot_fraction=1 / 32)
+        shape, tuned = t["run_kwargs"], tune_summary(t["report"])
+        log(f"bench: autotune chose {tuned['chosen']}")
     for i in range(args.warmup):
-        engine.run(spec, photons=photons, batches=batches, seed=1, arz=arz)
+        engine.run(spec, photons=photons, batches=batches, seed=1, arz=arz, **shape)
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     results = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        results.append(engine.run(spec, photons=photons, batches=batches, seed=1 + i, arz=arz))
+        results.append(engine.run(spec, photons=photons, batches=batches, seed=1 + i, arz=arz,
+                                  **shape))
         log(f"bench: step {i + 1}/{args.steps}: kernel {results[-1]['kernel_seconds'] * 1e3:.1f} ms")
     wall = time.perf_counter() - t0
 
@@ -231,11 +261,13 @@ def main() -> int:
                      "unit": "Tlane-op/s", "frac": achieved / PEAK_LANE_OPS, "traffic": traffic},
         "device": results[0]["device"],
         "launch": results[0]["launch"],
+        "autotune": tuned,
     }
     if not args.no_batched:
-        out["batched"] = bench_batched(engine, steps=2, warmup=1)
+        out["batched"] = bench_batched(engine, steps=2, warmup=1, tune=not args.no_tune)
     if not args.no_surrogate:
-        out["surrogate"] = bench_surrogate()
+        mc_rate = out["batched"]["photons_per_s"] if "batched" in out else None
+        out["surrogate"] = bench_surrogate(mc_rate)
     if not args.no_cpu:
         log("bench: timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mci, photons, batches)

@@ -154,6 +154,40 @@ void tmc_batched_destroy(tmc_batched_plan *plan);
 int tmc_hip_run_batched(const tmc_stack *stacks, int32_t n_stacks, const tmc_batched_opts *opts,
                         tmc_batched_out *out);
 
+/* ---------------------------------------------------------------- launch-shape autotuning
+ *
+ * No reference counterpart (the reference's CUDA engine launches one fixed shape,
+ * src/cuda/engine.cu). Tallies do not depend on the launch shape, so the shape is picked by
+ * timing pilot runs of the caller's own workload (every budget scaled by pilot_fraction, at
+ * least one photon per batch) at each candidate: waves per workgroup and workgroups per compute
+ * unit, which together decide how the LDS of a compute unit is split between the waves' tally
+ * slots and, for single runs, the workgroup's A(r,z) tile. Each candidate is timed three times;
+ * its fastest launch counts. */
+
+#define TMC_TUNE_MAX 16
+
+typedef struct tmc_tune_entry {
+    int32_t waves_per_group, groups, lds_bytes, arz_tile_z, arz_tile_r;
+    double kernel_seconds; /* fastest pilot launch */
+} tmc_tune_entry;
+
+typedef struct tmc_tune_report {
+    double pilot_fraction;
+    int32_t n_tried, best; /* tried[best] is the shape chosen */
+    tmc_tune_entry tried[TMC_TUNE_MAX];
+} tmc_tune_report;
+
+/* Single runs: copies *opts into *tuned with waves_per_group, groups and (with TMC_FLAG_ARZ) the
+ * A(r,z) tile set to the fastest candidate; the launch-shape fields of *opts are not used.
+ * pilot_fraction is in (0, 1]. report may be NULL. */
+int tmc_autotune(const tmc_spec *spec, const tmc_opts *opts, double pilot_fraction,
+                 tmc_opts *tuned, tmc_tune_report *report);
+
+/* Batched plans: times pilots of these stacks and rebuilds the plan's device state (and its
+ * graph) with the fastest shape, which later executes use. report may be NULL. */
+int tmc_batched_autotune(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
+                         uint64_t seed, double pilot_fraction, tmc_tune_report *report);
+
 /* Distribution of random layer stacks. Log-uniform: mua, mus, d, photons; uniform: g, n; the
  * layer count is uniform in [min_layers, max_layers]; the ambient index is fixed. */
 typedef struct tmc_stack_dist {

@@ -157,3 +157,10 @@ def test_cli_surface():
     assert r.returncode == 2 and "oracle" in r.stderr
     r = cli("run", "/nonexistent/x.mci")
     assert r.returncode == 1 and "tissuemc: error:" in r.stderr and "x.mci" in r.stderr
+
+
+def test_autotune_rejects_a_bad_pilot_fraction_before_any_device_work():
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    for f in (0.0, -1.0, 1.5, float("nan")):
+        with pytest.raises(engine.ConfigError, match="pilot_fraction"):
+            engine.autotune(spec, photons=1000, batches=10, pilot_fraction=f)

@@ -1,0 +1,73 @@
+"""Launch-shape autotuning on the GPU, through the C ABI.
+
+The tuner times pilot runs at candidate shapes and picks the fastest. What it must never do is
+change results: a run at the tuned shape gives the same integer tallies, bit for bit, as a run at
+the default shape, for single runs (with the A(r,z) tile, whose size follows from the shape) and
+for batched plans (whose device state and graph are rebuilt).
+"""
+
+import numpy as np
+import pytest
+
+import tissuemc_amd.engine as engine
+from conftest import load_spec
+
+pytestmark = pytest.mark.gpu
+
+GRID = dict(dz=0.01, dr=0.01, nz=100, nr=100)
+
+
+def check_report(rep):
+    tried = rep["tried"]
+    assert len(tried) >= 3
+    assert 0 <= rep["best"] < len(tried)
+    best = tried[rep["best"]]["kernel_seconds"]
+    assert best == min(t["kernel_seconds"] for t in tried)
+    assert all(t["kernel_seconds"] > 0 for t in tried)
+    assert len({(t["waves_per_group"], t["groups"]) for t in tried}) == len(tried)
+
+
+def test_single_run_tuned_shape_gives_identical_tallies(gpu):
+    spec = load_spec("v3_three_layer")
+    kw = dict(photons=400_000, batches=10, seed=5, arz=True)
+    t = engine.autotune(spec, pilot_fraction=0.25, **kw)
+    check_report(t["report"])
+    best = t["report"]["tried"][t["report"]["best"]]
+    assert t["run_kwargs"]["waves_per_group"] == best["waves_per_group"]
+    assert t["run_kwargs"]["groups"] == best["groups"]
+    assert list(t["run_kwargs"]["arz_tile"]) == best["arz_tile"]
+    a = engine.run(spec, **kw)
+    b = engine.run(spec, **kw, **t["run_kwargs"])
+    assert b["launch"]["waves_per_group"] == best["waves_per_group"]
+    assert b["launch"]["groups"] == best["groups"]
+    for key in ("rd", "tt", "absorbed", "az", "rr", "tr"):
+        assert np.array_equal(a["fixed"][key], b["fixed"][key]), key
+    assert np.array_equal(a["fixed"]["arz"], b["fixed"]["arz"])
+    assert a["events"] == b["events"] and a["boundary_steps"] == b["boundary_steps"]
+
+
+def test_batched_plan_autotune_keeps_results(gpu):
+    stacks = engine.generate_stacks(
+        33, 0, 96, engine.default_stack_dist(photons_lo=2000, photons_hi=30000))
+    with engine.BatchedPlan(len(stacks), batches_per_stack=4, **GRID) as plan:
+        before = plan.execute(stacks, seed=4)
+        rep = plan.autotune(stacks, seed=4, pilot_fraction=0.25)
+        check_report(rep)
+        after = plan.execute(stacks, seed=4)
+        best = rep["tried"][rep["best"]]
+        assert after["launch"]["waves_per_group"] == best["waves_per_group"]
+        assert after["launch"]["groups"] == best["groups"]
+        for key in ("rd", "tt", "absorbed", "events", "anomalies", "boundary_steps", "az", "rr",
+                    "tr"):
+            assert np.array_equal(before[key], after[key]), key
+        # the rebuilt plan still replays its graph on new inputs
+        again = plan.execute(stacks, seed=5)
+        assert not np.array_equal(again["rd"], after["rd"])
+
+
+def test_pilot_fraction_is_checked(gpu):
+    spec = load_spec("v1_slab")
+    with pytest.raises(engine.ConfigError, match="pilot_fraction"):
+        engine.autotune(spec, photons=1000, batches=10, pilot_fraction=0.0)
+    with pytest.raises(engine.ConfigError, match="pilot_fraction"):
+        engine.autotune(spec, photons=1000, batches=10, pilot_fraction=1.5)

@@ -277,6 +277,22 @@ int tmc_batched_create(int32_t n_stacks, const tmc_batched_opts *opts, tmc_batch
     });
 }
 
+int tmc_autotune(const tmc_spec *spec, const tmc_opts *opts, double pilot_fraction,
+                 tmc_opts *tuned, tmc_tune_report *report) {
+    return guarded([&] {
+        if (!spec || !opts || !tuned) config_fail("spec, opts and tuned must not be NULL");
+        tmc::autotune_single(*spec, *opts, pilot_fraction, tuned, report);
+    });
+}
+
+int tmc_batched_autotune(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
+                         uint64_t seed, double pilot_fraction, tmc_tune_report *report) {
+    return guarded([&] {
+        if (!plan || !plan->impl || !stacks) config_fail("plan and stacks must not be NULL");
+        tmc::batched_autotune(*plan->impl, stacks, n_stacks, seed, pilot_fraction, report);
+    });
+}
+
 int tmc_batched_execute(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
                         uint64_t seed, tmc_batched_out *out) {
     return guarded([&] {

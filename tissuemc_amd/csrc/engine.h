@@ -32,6 +32,7 @@ void run_transport(const tmc_spec &spec, const tmc_opts &opts, tmc_out *out);
 struct tmc_tuning {
     int waves_per_group = 0, groups = 0, chunk = 0, arz_tile_z = -1, arz_tile_r = -1;
     uint64_t step_limit = 0;
+    size_t lds_budget = 0;  // LDS one workgroup may use; 0 = the device's per-workgroup limit
 };
 
 // Batched runs (opaque to callers of the C ABI).
@@ -40,6 +41,12 @@ BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &opts);
 void batched_execute(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, uint64_t seed,
                      tmc_batched_out *out);
 void batched_destroy(BatchedPlan *plan);
+
+// Launch-shape autotuning by pilot runs (tmc_autotune, tmc_batched_autotune).
+void autotune_single(const tmc_spec &spec, const tmc_opts &opts, double pilot_fraction,
+                     tmc_opts *tuned, tmc_tune_report *report);
+void batched_autotune(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, uint64_t seed,
+                      double pilot_fraction, tmc_tune_report *report);
 
 // The surrogate model (mlp.hip).
 class MlpModel;

@@ -686,6 +686,7 @@ struct DeviceInfo {
     int id = 0;
     int cus = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
     std::string name;
 };
 
@@ -703,6 +704,8 @@ DeviceInfo device_info(int device) {
     d.id = device;
     d.cus = prop.multiProcessorCount;
     d.lds_per_block = prop.sharedMemPerBlock;
+    d.lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                         : prop.sharedMemPerBlock;
     d.name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " +
              std::to_string(prop.multiProcessorCount) + " CUs)";
     return d;
@@ -747,7 +750,8 @@ class Resident {
         a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
         a_.wave_words = ((a_.slot_words + 1) & ~1) + p.lmax * kLayerFields / 2;
 
-        const size_t cap = dev.lds_per_block;
+        const size_t cap = t.lds_budget > 0 ? std::min<size_t>(t.lds_budget, dev.lds_per_block)
+                                            : dev.lds_per_block;
         int wpg = t.waves_per_group > 0 ? t.waves_per_group : 16;
         if (wpg > 16) throw EngineError(TMC_ERR_CONFIG, "waves_per_group must be <= 16");
         auto wave_bytes = [&](int w) { return static_cast<size_t>(w) * a_.wave_words * 8; };
@@ -954,8 +958,10 @@ class Resident {
 
 // ------------------------------------------------------------------ single run
 
-void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
-    const auto t0 = std::chrono::steady_clock::now();
+namespace {
+
+// Checks a single run's spec and options; returns the spec with the photon override applied.
+tmc_spec checked_single(const tmc_spec &spec_in, const tmc_opts &opt) {
     tmc_spec spec = spec_in;
     if (opt.photons != 0) spec.photons = opt.photons;
     validate_spec(spec);
@@ -963,13 +969,13 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     if (static_cast<uint64_t>(opt.batches) > spec.photons)
         throw EngineError(TMC_ERR_CONFIG, "batches cannot exceed the number of photons");
     const bool profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
-    const bool want_arz = (opt.flags & TMC_FLAG_ARZ) != 0;
-    if (want_arz && !profiles)
+    if ((opt.flags & TMC_FLAG_ARZ) != 0 && !profiles)
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) is a profile: it cannot be combined with no-profiles");
-    if (want_arz && out->arz == nullptr)
-        throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
-    const DeviceInfo dev = device_info(opt.device);
+    return spec;
+}
 
+// One stack cut into the reference's batches (BatchPlan::first_of), one unit per batch.
+Problem single_problem(const tmc_spec &spec, const tmc_opts &opt) {
     Problem p;
     const int L = spec.num_layers;
     const uint64_t B = static_cast<uint64_t>(opt.batches);
@@ -977,8 +983,8 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     p.abs_stride = L;
     p.nz = spec.nz;
     p.nr = spec.nr;
-    p.profiles = profiles;
-    p.arz = want_arz;
+    p.profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
+    p.arz = (opt.flags & TMC_FLAG_ARZ) != 0;
     p.photons = spec.photons;
     p.n_srows = B;
     p.n_prows = B;
@@ -992,7 +998,7 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     const uint64_t base = spec.photons / B, extra = spec.photons % B;
     for (uint64_t b = 0; b < B; ++b) {
         UnitRec u;
-        u.first = b * base + (b < extra ? b : extra);  // the reference's BatchPlan::first_of
+        u.first = b * base + (b < extra ? b : extra);
         u.count = base + (b < extra ? 1 : 0);
         u.stack = 0;
         u.prow = static_cast<uint32_t>(b);
@@ -1000,6 +1006,10 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
         u.word3 = opt.stream;
         p.units.push_back(u);
     }
+    return p;
+}
+
+tmc_tuning tuning_of(const tmc_opts &opt) {
     tmc_tuning t;
     t.waves_per_group = opt.waves_per_group;
     t.groups = opt.groups;
@@ -1007,8 +1017,21 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     t.arz_tile_z = opt.arz_tile_z;
     t.arz_tile_r = opt.arz_tile_r;
     t.step_limit = opt.step_limit;
+    return t;
+}
 
-    Resident r(dev, p, t);
+}  // namespace
+
+void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const tmc_spec spec = checked_single(spec_in, opt);
+    const bool want_arz = (opt.flags & TMC_FLAG_ARZ) != 0;
+    if (want_arz && out->arz == nullptr)
+        throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
+    const DeviceInfo dev = device_info(opt.device);
+
+    const Problem p = single_problem(spec, opt);
+    Resident r(dev, p, tuning_of(opt));
     r.upload(p, opt.seed);
     out->kernel_seconds = r.run(false);
     r.download(out->rd, out->tt, out->absorbed, out->events, out->anomalies, out->boundary_steps,
@@ -1151,6 +1174,133 @@ void batched_execute(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, u
 }
 
 void batched_destroy(BatchedPlan *plan) { delete plan; }
+
+// ------------------------------------------------------------------ launch-shape autotuning
+
+namespace {
+
+// Candidate shapes: waves per workgroup x workgroups per compute unit. One 16-wave workgroup
+// gives one workgroup-wide A(r,z) tile the most LDS; more, smaller workgroups split the LDS and
+// spread the claim traffic. 4 waves per SIMD is what the transport kernel's registers allow with
+// one workgroup per CU; 20 waves per CU (5 per SIMD) is the register limit.
+struct Shape {
+    int wpg, per_cu;
+};
+constexpr Shape kShapes[] = {{16, 1}, {12, 1}, {8, 1}, {8, 2}, {10, 2}, {6, 2},
+                             {4, 4},  {5, 4},  {4, 2}, {4, 5}};
+
+double check_fraction(double f) {
+    if (!(f > 0.0) || f > 1.0) throw EngineError(TMC_ERR_CONFIG, "pilot_fraction must be in (0, 1]");
+    return f;
+}
+
+uint64_t scaled_budget(uint64_t photons, double f, uint64_t at_least) {
+    const double v = std::ceil(static_cast<double>(photons) * f);
+    return std::max<uint64_t>(at_least, static_cast<uint64_t>(v));
+}
+
+// Times `pilot` at every candidate shape; returns the fastest tuning (base plus shape) and
+// fills the report. The tile of a single run follows from the LDS budget of the shape.
+tmc_tuning tune_shapes(const DeviceInfo &dev, const Problem &pilot, const tmc_tuning &base,
+                       uint64_t seed, double fraction, tmc_tune_report *rep) {
+    tmc_tune_report local;
+    tmc_tune_report &r = rep ? *rep : local;
+    std::memset(&r, 0, sizeof r);
+    r.pilot_fraction = fraction;
+    r.best = -1;
+    tmc_tuning best = base;
+    double best_t = 0.0;
+    for (const Shape &sh : kShapes) {
+        if (r.n_tried >= TMC_TUNE_MAX) break;
+        tmc_tuning t = base;
+        t.waves_per_group = sh.wpg;
+        t.groups = dev.cus * sh.per_cu;
+        t.lds_budget = dev.lds_per_cu / static_cast<size_t>(sh.per_cu);
+        t.arz_tile_z = t.arz_tile_r = -1;
+        std::unique_ptr<Resident> res;
+        try {
+            res = std::make_unique<Resident>(dev, pilot, t);
+        } catch (const EngineError &e) {
+            if (e.code() == TMC_ERR_CONFIG) continue;  // does not fit this shape's LDS budget
+            throw;
+        }
+        if (res->waves_per_group() != sh.wpg) continue;  // the slots did not fit: shrunk
+        res->upload(pilot, seed);
+        double tmin = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double dt = res->run(false);
+            tmin = k == 0 ? dt : std::min(tmin, dt);
+        }
+        tmc_tune_entry &e = r.tried[r.n_tried];
+        e.waves_per_group = sh.wpg;
+        e.groups = t.groups;
+        e.lds_bytes = static_cast<int32_t>(res->lds_bytes());
+        e.arz_tile_z = res->tile_z();
+        e.arz_tile_r = res->tile_r();
+        e.kernel_seconds = tmin;
+        if (r.best < 0 || tmin < best_t) {
+            r.best = r.n_tried;
+            best_t = tmin;
+            best = t;
+            if (pilot.arz) {
+                best.arz_tile_z = res->tile_z();
+                best.arz_tile_r = res->tile_r();
+            }
+        }
+        ++r.n_tried;
+    }
+    if (r.best < 0) throw EngineError(TMC_ERR_CONFIG, "no launch shape fits this problem's LDS tallies");
+    return best;
+}
+
+}  // namespace
+
+void autotune_single(const tmc_spec &spec_in, const tmc_opts &opt, double pilot_fraction,
+                     tmc_opts *tuned, tmc_tune_report *report) {
+    const double f = check_fraction(pilot_fraction);
+    const tmc_spec spec = checked_single(spec_in, opt);
+    const DeviceInfo dev = device_info(opt.device);
+    tmc_spec ps = spec;
+    ps.photons = scaled_budget(spec.photons, f, static_cast<uint64_t>(opt.batches));
+    tmc_opts po = opt;
+    po.photons = 0;
+    const Problem pilot = single_problem(ps, po);
+    const tmc_tuning best = tune_shapes(dev, pilot, tuning_of(opt), opt.seed, f, report);
+    *tuned = opt;
+    tuned->waves_per_group = best.waves_per_group;
+    tuned->groups = best.groups;
+    tuned->arz_tile_z = best.arz_tile_z;
+    tuned->arz_tile_r = best.arz_tile_r;
+}
+
+void batched_autotune(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, uint64_t seed,
+                      double pilot_fraction, tmc_tune_report *report) {
+    const double f = check_fraction(pilot_fraction);
+    if (static_cast<size_t>(n_stacks) != plan.shape.n_prows)
+        throw EngineError(TMC_ERR_CONFIG, "the plan was made for " + std::to_string(plan.shape.n_prows) +
+                                              " stacks, got " + std::to_string(n_stacks));
+    const uint64_t Bs = static_cast<uint64_t>(plan.opt.batches_per_stack);
+    std::vector<tmc_stack> ps(stacks, stacks + n_stacks);
+    for (int s = 0; s < n_stacks; ++s) {
+        check_stack(stacks[s], s);
+        if (stacks[s].photons < Bs)
+            throw EngineError(TMC_ERR_CONFIG, "stack " + std::to_string(s) +
+                                                  ": batches cannot exceed the number of photons");
+        ps[static_cast<size_t>(s)].photons = scaled_budget(stacks[s].photons, f, Bs);
+    }
+    Problem pilot = plan.shape;
+    batched_units(ps.data(), n_stacks, plan.opt.batches_per_stack, pilot);
+    batched_stacks(ps.data(), n_stacks, plan.opt, pilot);
+    tmc_tuning base;
+    base.chunk = plan.opt.chunk;
+    base.step_limit = plan.opt.step_limit;
+    base.arz_tile_z = base.arz_tile_r = 0;
+    const tmc_tuning best = tune_shapes(plan.dev, pilot, base, seed, f, report);
+    plan.res.reset();
+    plan.res = std::make_unique<Resident>(plan.dev, plan.shape, best);
+    plan.opt.waves_per_group = best.waves_per_group;
+    plan.opt.groups = best.groups;
+}
 
 // ------------------------------------------------------------------ parity hook
 

@@ -90,6 +90,19 @@ class TmcStackDist(C.Structure):
                 ("n_ambient", C.c_double)]
 
 
+TUNE_MAX = 16
+
+
+class TmcTuneEntry(C.Structure):
+    _fields_ = [("waves_per_group", C.c_int32), ("groups", C.c_int32), ("lds_bytes", C.c_int32),
+                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("kernel_seconds", C.c_double)]
+
+
+class TmcTuneReport(C.Structure):
+    _fields_ = [("pilot_fraction", C.c_double), ("n_tried", C.c_int32), ("best", C.c_int32),
+                ("tried", TmcTuneEntry * TUNE_MAX)]
+
+
 EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_point_unit",
             "tmc_default_opts", "tmc_validate_spec", "tmc_parse_mci_file", "tmc_parse_mci_text",
             "tmc_hip_philox_stream", "tmc_batched_create", "tmc_batched_execute",
@@ -97,7 +110,7 @@ EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_poi
             "tmc_generate_stacks", "tmc_mlp_default_config", "tmc_mlp_create", "tmc_mlp_destroy",
             "tmc_mlp_param_count", "tmc_mlp_set_data", "tmc_mlp_train", "tmc_mlp_gradients",
             "tmc_mlp_predict", "tmc_mlp_get_params", "tmc_mlp_set_params", "tmc_mlp_set_lr",
-            "tmc_mlp_flops")
+            "tmc_mlp_flops", "tmc_autotune", "tmc_batched_autotune")
 
 _lib: C.CDLL | None = None
 
@@ -142,6 +155,12 @@ def load() -> C.CDLL:
     L.tmc_generate_stacks.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(TmcStackDist),
                                       C.POINTER(TmcStack)]
     L.tmc_generate_stacks.restype = C.c_int
+    L.tmc_autotune.argtypes = [C.POINTER(TmcSpec), C.POINTER(TmcOpts), C.c_double,
+                               C.POINTER(TmcOpts), C.POINTER(TmcTuneReport)]
+    L.tmc_autotune.restype = C.c_int
+    L.tmc_batched_autotune.argtypes = [C.c_void_p, C.POINTER(TmcStack), C.c_int32, C.c_uint64,
+                                       C.c_double, C.POINTER(TmcTuneReport)]
+    L.tmc_batched_autotune.restype = C.c_int
     _lib = L
     return L
 
@@ -222,6 +241,48 @@ def batch_estimate(sums, photons) -> tuple[float, float]:
     return mean, float(np.sqrt(((per - per.mean()) ** 2).sum() / (n - 1) / n))
 
 
+def _opts(*, seed, batches, device, photons, stream, arz, profiles, waves_per_group=0, groups=0,
+          chunk=0, arz_tile=None, step_limit=0) -> TmcOpts:
+    o = TmcOpts()
+    load().tmc_default_opts(C.byref(o))
+    o.seed, o.batches, o.device, o.photons, o.stream = seed, int(batches), device, photons, stream
+    o.flags = (FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
+    o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
+    if arz_tile is not None:
+        o.arz_tile_z, o.arz_tile_r = arz_tile
+    return o
+
+
+def _report(r: TmcTuneReport) -> dict:
+    tried = [{"waves_per_group": e.waves_per_group, "groups": e.groups, "lds_bytes": e.lds_bytes,
+              "arz_tile": [e.arz_tile_z, e.arz_tile_r], "kernel_seconds": e.kernel_seconds}
+             for e in r.tried[:r.n_tried]]
+    return {"pilot_fraction": r.pilot_fraction, "best": r.best, "tried": tried}
+
+
+def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
+             arz: bool = False, profiles: bool = True, stream: int = 0, chunk: int = 0,
+             step_limit: int = 0, pilot_fraction: float = 1 / 32) -> dict:
+    """Pick the launch shape of a single run by timing pilot runs (tmc_autotune).
+
+    Returns ``{"run_kwargs": ..., "report": ...}``; ``run(spec, **run_kwargs, ...)`` uses the
+    shape chosen. Tallies do not depend on it.
+    """
+    L = load()
+    cs = spec_to_c(spec)
+    o = _opts(seed=seed, batches=batches, device=device, photons=photons or int(spec["photons"]),
+              stream=stream, arz=arz, profiles=profiles, chunk=chunk, step_limit=step_limit)
+    tuned, rep = TmcOpts(), TmcTuneReport()
+    code = L.tmc_autotune(C.byref(cs), C.byref(o), float(pilot_fraction), C.byref(tuned),
+                          C.byref(rep))
+    if code != TMC_OK:
+        _raise(code)
+    kw = {"waves_per_group": tuned.waves_per_group, "groups": tuned.groups}
+    if arz:
+        kw["arz_tile"] = (tuned.arz_tile_z, tuned.arz_tile_r)
+    return {"run_kwargs": kw, "report": _report(rep)}
+
+
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
         arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
         groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
@@ -235,13 +296,9 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     cs = spec_to_c(spec)
     n_ph = photons or int(spec["photons"])
     B, NL, nz, nr = int(batches), cs.num_layers, cs.nz, cs.nr
-    o = TmcOpts()
-    L.tmc_default_opts(C.byref(o))
-    o.seed, o.batches, o.device, o.photons, o.stream = seed, B, device, n_ph, stream
-    o.flags = (FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
-    o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
-    if arz_tile is not None:
-        o.arz_tile_z, o.arz_tile_r = arz_tile
+    o = _opts(seed=seed, batches=B, device=device, photons=n_ph, stream=stream, arz=arz,
+              profiles=profiles, waves_per_group=waves_per_group, groups=groups, chunk=chunk,
+              arz_tile=arz_tile, step_limit=step_limit)
     if B < 1:
         raise ConfigError("batches must be >= 1")
     rd, tt, ev, an, bs = (np.zeros(B, np.uint64) for _ in range(5))
@@ -390,6 +447,17 @@ class BatchedPlan:
         code = load().tmc_batched_create(n_stacks, C.byref(o), C.byref(self._h))
         if code != TMC_OK:
             _raise(code)
+
+    def autotune(self, stacks, seed: int = 1, pilot_fraction: float = 1 / 16) -> dict:
+        """Time pilot runs of these stacks at each candidate launch shape and keep the fastest
+        for later executes (tmc_batched_autotune); returns the report."""
+        arr = stacks_array(stacks)
+        rep = TmcTuneReport()
+        code = load().tmc_batched_autotune(self._h, arr, len(arr), seed, float(pilot_fraction),
+                                           C.byref(rep))
+        if code != TMC_OK:
+            _raise(code)
+        return _report(rep)
 
     def execute(self, stacks, seed: int = 1) -> dict:
         arr = stacks_array(stacks)
