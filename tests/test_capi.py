@@ -155,6 +155,9 @@ def test_cli_surface():
     assert r.returncode == 2
     r = cli("run", str(CONFIGS / "v1_slab.mci"), "--engine", "cpu")
     assert r.returncode == 2 and "oracle" in r.stderr
+    for bad in ("0", "1.5", "x"):
+        r = cli("run", str(CONFIGS / "v1_slab.mci"), "--autotune", bad)
+        assert r.returncode == 2 and "fraction" in r.stderr
     r = cli("run", "/nonexistent/x.mci")
     assert r.returncode == 1 and "tissuemc: error:" in r.stderr and "x.mci" in r.stderr
 

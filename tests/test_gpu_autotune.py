@@ -71,3 +71,23 @@ def test_pilot_fraction_is_checked(gpu):
         engine.autotune(spec, photons=1000, batches=10, pilot_fraction=0.0)
     with pytest.raises(engine.ConfigError, match="pilot_fraction"):
         engine.autotune(spec, photons=1000, batches=10, pilot_fraction=1.5)
+
+
+def test_cli_autotune(gpu, tmp_path):
+    import json
+    import subprocess
+
+    from conftest import CONFIGS, ROOT
+
+    cli = ROOT / "tissuemc_amd" / "bin" / "tissuemc"
+    base = [str(cli), "run", str(CONFIGS / "v1_slab.mci"), "--photons", "200000", "--batches", "10"]
+    a = subprocess.run(base + ["--output", str(tmp_path / "a.json")], capture_output=True, text=True)
+    b = subprocess.run(base + ["--autotune", "0.25", "--output", str(tmp_path / "b.json")],
+                       capture_output=True, text=True)
+    assert a.returncode == 0, a.stderr
+    assert b.returncode == 0, b.stderr
+    assert "autotune chose" in b.stdout
+    ra = json.loads((tmp_path / "a.json").read_text())["runs"][0]
+    rb = json.loads((tmp_path / "b.json").read_text())["runs"][0]
+    assert [x["rd"] for x in ra["batches"]] == [x["rd"] for x in rb["batches"]]
+    assert ra["a_z"] == rb["a_z"] and ra["events"] == rb["events"]

@@ -32,6 +32,8 @@ void usage(std::FILE *to) {
                  "  --chunk C            photon ids claimed per wave at a time\n"
                  "  --arz-tile Z R       A(r,z) bins kept in LDS (default automatic)\n"
                  "  --stream W           Philox counter word 3 (default 0, the reference streams)\n"
+                 "  --autotune F         pick the launch shape from pilot runs at fraction F of the\n"
+                 "                       photons (0 < F <= 1); results do not depend on the shape\n"
                  "  --output FILE        write full results as JSON\n");
 }
 
@@ -40,6 +42,16 @@ unsigned long long parse_count(const char *name, const char *text) {
     const unsigned long long v = std::strtoull(text, &end, 10);
     if (end == text || *end != '\0' || text[0] == '-') {
         std::fprintf(stderr, "tissuemc: %s expects a non-negative integer, got '%s'\n", name, text);
+        std::exit(2);
+    }
+    return v;
+}
+
+double parse_fraction(const char *name, const char *text) {
+    char *end = nullptr;
+    const double v = std::strtod(text, &end);
+    if (end == text || *end != '\0' || !(v > 0.0) || v > 1.0) {
+        std::fprintf(stderr, "tissuemc: %s expects a fraction in (0, 1], got '%s'\n", name, text);
         std::exit(2);
     }
     return v;
@@ -54,6 +66,7 @@ int run_command(int argc, char **argv) {
     tmc_opts opt;
     tmc_default_opts(&opt);
     std::string output;
+    double tune_fraction = 0.0;
     for (int i = 3; i < argc; ++i) {
         const std::string a = argv[i];
         auto value = [&](const char *name) -> const char * {
@@ -101,6 +114,8 @@ int run_command(int argc, char **argv) {
             opt.arz_tile_r = static_cast<int32_t>(parse_count(a.c_str(), value(a.c_str())));
         } else if (a == "--stream") {
             opt.stream = static_cast<uint32_t>(parse_count(a.c_str(), value(a.c_str())));
+        } else if (a == "--autotune") {
+            tune_fraction = parse_fraction(a.c_str(), value(a.c_str()));
         } else if (a == "--output") {
             output = value("--output");
         } else {
@@ -126,7 +141,19 @@ int run_command(int argc, char **argv) {
     }
     int status = 0;
     for (size_t r = 0; r < runs.size(); ++r) {
-        const tmc::RunResult res = tmc::run_hip(runs[r], opt);
+        tmc_opts run_opt = opt;
+        if (tune_fraction > 0.0) {
+            tmc_tune_report rep;
+            const int code = tmc_autotune(&runs[r], &opt, tune_fraction, &run_opt, &rep);
+            if (code == TMC_ERR_CONFIG) throw tmc::ConfigError(tmc_last_error());
+            if (code != TMC_OK) throw std::runtime_error(tmc_last_error());
+            const tmc_tune_entry &b = rep.tried[rep.best];
+            std::printf("run %zu: autotune chose %d waves x %d workgroups (%d shapes, pilot %.3g, "
+                        "%.2f ms)\n",
+                        r + 1, b.waves_per_group, b.groups, rep.n_tried, tune_fraction,
+                        b.kernel_seconds * 1e3);
+        }
+        const tmc::RunResult res = tmc::run_hip(runs[r], run_opt);
         const auto rd = res.rd(), tt = res.tt(), ab = res.absorbed();
         const auto cons = res.conservation_residual();
         std::printf("run %zu: %llu photons, %zu batches, %.2f s (%s, %s)\n", r + 1,
