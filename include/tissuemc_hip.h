@@ -45,6 +45,7 @@ enum {
 #define TMC_FLAG_ARZ 1u           /* also tally A(r, z) into tmc_out.arz */
 #define TMC_FLAG_NO_PROFILES 2u   /* diagnostic: skip the A_z/R_r/T_r/A_rz profiles (zeros) */
 #define TMC_FLAG_NO_GRAPH 4u      /* batched runs: launch directly instead of replaying a graph */
+#define TMC_FLAG_ANGLES 8u        /* also tally the exit-angle profiles into tmc_out.ra/ta/rra/tra */
 
 typedef struct tmc_layer {
     double n;   /* refractive index */
@@ -58,7 +59,7 @@ typedef struct tmc_layer {
 typedef struct tmc_spec {
     uint64_t photons;
     double dz, dr;          /* grid bin widths [cm] */
-    int32_t nz, nr, na;     /* grid bin counts (na is accepted and not used) */
+    int32_t nz, nr, na;     /* grid bin counts (na: exit-angle bins, used by TMC_FLAG_ANGLES) */
     double n_above, n_below;
     int32_t num_layers;
     tmc_layer layers[TMC_MAX_LAYERS];
@@ -98,6 +99,10 @@ typedef struct tmc_out {
     double wall_seconds;      /* whole call */
     char device[96];          /* device name and architecture */
     int32_t groups, waves_per_group, lds_bytes, chunk, arz_tile_z, arz_tile_r;
+    /* With TMC_FLAG_ANGLES (no reference counterpart; MCML's Rd_a, Tt_a, Rd_ra, Tt_ra): weight
+     * escaping at an angle from the normal in na bins of pi / (2 na), all batches. */
+    uint64_t *ra, *ta;        /* [na]    diffuse reflectance, transmittance per angle bin */
+    uint64_t *rra, *tra;      /* [nr*na] per (radius, angle) bin, row-major in radius */
 } tmc_out;
 
 /* ---------------------------------------------------------------- batched multi-stack runs

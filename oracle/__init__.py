@@ -50,7 +50,8 @@ _PD = C.POINTER(C.c_double)
 class OrcOut(C.Structure):
     _fields_ = [("first", _P64), ("count", _P64), ("rd", _PD), ("tt", _PD), ("absorbed", _PD),
                 ("events", _P64), ("anomalies", _P64), ("boundary_steps", _P64), ("az", _PD),
-                ("rr", _PD), ("tr", _PD), ("arz", _PD), ("specular", C.c_double),
+                ("rr", _PD), ("tr", _PD), ("arz", _PD), ("ra", _PD), ("ta", _PD), ("rra", _PD),
+                ("tra", _PD), ("specular", C.c_double),
                 ("threads_used", C.c_int32), ("wall_seconds", C.c_double)]
 
 
@@ -146,8 +147,12 @@ def _ptr(a: np.ndarray, t):
 
 
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threads: int = 0,
-        precision: str = "float64", stream: int = 0, arz: bool = False) -> dict:
-    """Simulate one run with the oracle; returns a result dict in the reference's schema."""
+        precision: str = "float64", stream: int = 0, arz: bool = False,
+        angles: bool = False) -> dict:
+    """Simulate one run with the oracle; returns a result dict in the reference's schema.
+
+    With ``angles`` the exit-angle profiles are added: ``rd_a``/``t_a`` [na] and ``rd_ra``/
+    ``t_ra`` [nr, na] (sums over batches), per batch under ``ra_b``/``ta_b``."""
     s = to_spec(spec)
     n_ph = photons or s.photons
     B, L, nz, nr = batches, s.num_layers, s.nz, s.nr
@@ -165,12 +170,19 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threa
     rr = np.zeros(B * nr)
     tr = np.zeros(B * nr)
     arz_b = np.zeros(B * nz * nr) if arz else None
+    na = max(int(s.na), 1)
+    ang = [np.zeros(B * na), np.zeros(B * na), np.zeros(B * nr * na), np.zeros(B * nr * na)] \
+        if angles else [None] * 4
     out = OrcOut(first=_ptr(first, C.c_uint64), count=_ptr(count, C.c_uint64),
                  rd=_ptr(rd, C.c_double), tt=_ptr(tt, C.c_double),
                  absorbed=_ptr(absorbed, C.c_double), events=_ptr(events, C.c_uint64),
                  anomalies=_ptr(anomalies, C.c_uint64), boundary_steps=_ptr(bsteps, C.c_uint64),
                  az=_ptr(az, C.c_double), rr=_ptr(rr, C.c_double), tr=_ptr(tr, C.c_double),
-                 arz=_ptr(arz_b, C.c_double) if arz else None)
+                 arz=_ptr(arz_b, C.c_double) if arz else None,
+                 ra=_ptr(ang[0], C.c_double) if angles else None,
+                 ta=_ptr(ang[1], C.c_double) if angles else None,
+                 rra=_ptr(ang[2], C.c_double) if angles else None,
+                 tra=_ptr(ang[3], C.c_double) if angles else None)
     err = C.create_string_buffer(256)
     if lib().orc_run(C.byref(s), C.byref(o), C.byref(out), err, 256) != 0:
         raise ValueError(err.value.decode())
@@ -204,6 +216,11 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threa
         arz3 = arz_b.reshape(B, nz, nr)
         res["a_rz_b"] = arz3
         res["a_rz"] = arz3.sum(axis=0)
+    if angles:
+        res["ra_b"], res["ta_b"] = ang[0].reshape(B, na), ang[1].reshape(B, na)
+        res["rd_a"], res["t_a"] = res["ra_b"].sum(axis=0), res["ta_b"].sum(axis=0)
+        res["rd_ra"] = ang[2].reshape(B, nr, na).sum(axis=0)
+        res["t_ra"] = ang[3].reshape(B, nr, na).sum(axis=0)
     return res
 
 

@@ -76,6 +76,8 @@ typedef struct {
     double *az;                        /* [B*nz] */
     double *rr, *tr;                   /* [B*nr] */
     double *arz;                       /* [B*nz*nr] */
+    double *ra, *ta;                   /* [B*na] exit-angle profiles, or NULL */
+    double *rra, *tra;                 /* [B*nr*na] (radius, angle), or NULL with ra */
     double specular;
     int32_t threads_used;
     double wall_seconds;
@@ -190,6 +192,7 @@ typedef struct {
     csum rd, tt;
     csum absorbed[ORC_MAX_LAYERS];
     csum *az, *rr, *tr, *arz;
+    csum *ra, *ta, *rra, *tra;
     uint64_t events, anomalies, boundary_steps;
 } orc_tally;
 
@@ -306,6 +309,13 @@ static void run_batch(orc_job *job, uint64_t b) {
     t.rr = calloc((size_t)nr, sizeof(csum));
     t.tr = calloc((size_t)nr, sizeof(csum));
     if (out->arz) t.arz = calloc((size_t)nz * (size_t)nr, sizeof(csum));
+    const int na = run->na;
+    if (out->ra) {
+        t.ra = calloc((size_t)na, sizeof(csum));
+        t.ta = calloc((size_t)na, sizeof(csum));
+        t.rra = calloc((size_t)nr * (size_t)na, sizeof(csum));
+        t.tra = calloc((size_t)nr * (size_t)na, sizeof(csum));
+    }
     const uint64_t first = plan_first(&job->plan, b), count = plan_count(&job->plan, b);
     for (uint64_t id = first; id < first + count; ++id) {
         orc_rng rng;
@@ -333,10 +343,25 @@ static void run_batch(orc_job *job, uint64_t b) {
         const size_t n = (size_t)nz * (size_t)nr;
         for (size_t i = 0; i < n; ++i) out->arz[b * n + i] = csum_value(&t.arz[i]);
     }
+    if (out->ra) {
+        for (int i = 0; i < na; ++i) {
+            out->ra[b * (uint64_t)na + (uint64_t)i] = csum_value(&t.ra[i]);
+            out->ta[b * (uint64_t)na + (uint64_t)i] = csum_value(&t.ta[i]);
+        }
+        const size_t n = (size_t)nr * (size_t)na;
+        for (size_t i = 0; i < n; ++i) {
+            out->rra[b * n + i] = csum_value(&t.rra[i]);
+            out->tra[b * n + i] = csum_value(&t.tra[i]);
+        }
+    }
     free(t.az);
     free(t.rr);
     free(t.tr);
     free(t.arz);
+    free(t.ra);
+    free(t.ta);
+    free(t.rra);
+    free(t.tra);
 }
 
 static void *worker(void *arg) {
@@ -366,6 +391,9 @@ int orc_run(const orc_spec *spec_in, const orc_opts *opt, orc_out *out, char *er
         return fail_msg(err, errlen, "threads must be in [0, 1024] (0 = one per hardware thread)");
     if (opt->precision != 64 && opt->precision != 32)
         return fail_msg(err, errlen, "precision must be 64 or 32");
+    if (out->ra && (!out->ta || !out->rra || !out->tra))
+        return fail_msg(err, errlen, "angle profiles need ra, ta, rra and tra");
+    if (out->ra && run.na < 1) return fail_msg(err, errlen, "angle profiles need na >= 1");
     int threads = opt->threads;
     if (threads == 0) {
         const long hw = sysconf(_SC_NPROCESSORS_ONLN);

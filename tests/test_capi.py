@@ -167,3 +167,15 @@ def test_autotune_rejects_a_bad_pilot_fraction_before_any_device_work():
     for f in (0.0, -1.0, 1.5, float("nan")):
         with pytest.raises(engine.ConfigError, match="pilot_fraction"):
             engine.autotune(spec, photons=1000, batches=10, pilot_fraction=f)
+
+
+def test_angle_profiles_are_checked_before_any_device_work():
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    with pytest.raises(engine.ConfigError, match="no-profiles"):
+        engine.run(spec, photons=100, batches=1, angles=True, profiles=False)
+    L = engine.load()
+    o = engine.TmcBatchedOpts(seed=1, batches_per_stack=1, dz=0.01, dr=0.01, nz=10, nr=10,
+                              flags=engine.FLAG_ANGLES)
+    plan = engine.C.c_void_p()
+    assert L.tmc_batched_create(4, engine.C.byref(o), engine.C.byref(plan)) == engine.TMC_ERR_CONFIG
+    assert "single-run options" in L.tmc_last_error().decode()

@@ -50,7 +50,8 @@ def _sums(run: dict, key: str) -> list[float]:
 
 
 def profile_gaps(a: dict, b: dict, key: str) -> list[Gap]:
-    pa, pb = list(a.get(key) or []), list(b.get(key) or [])
+    pa = [float(x) for x in (a.get(key) if a.get(key) is not None else [])]
+    pb = [float(x) for x in (b.get(key) if b.get(key) is not None else [])]
     if not pa or not pb or len(pa) != len(pb):
         return []
     total = math.fsum(pa)

@@ -26,6 +26,9 @@ void usage(std::FILE *to) {
                  "  --precision P        float32 (the GPU transport precision)\n"
                  "  --device D           HIP device ordinal (default 0)\n"
                  "  --arz                also tally A(r,z)\n"
+                 "  --angles             also tally the exit-angle profiles (na bins of the .mci)\n"
+                 "  --mco                write MCML's .mco file for each run, named as in the .mci\n"
+                 "                       (implies --arz --angles)\n"
                  "  --no-profiles        diagnostic: skip the depth and radius profiles\n"
                  "  --waves-per-group W  launch shape (default automatic)\n"
                  "  --groups G           persistent workgroups (default one per compute unit)\n"
@@ -67,6 +70,7 @@ int run_command(int argc, char **argv) {
     tmc_default_opts(&opt);
     std::string output;
     double tune_fraction = 0.0;
+    bool write_mco_files = false;
     for (int i = 3; i < argc; ++i) {
         const std::string a = argv[i];
         auto value = [&](const char *name) -> const char * {
@@ -101,6 +105,11 @@ int run_command(int argc, char **argv) {
             opt.device = static_cast<int32_t>(parse_count("--device", value("--device")));
         } else if (a == "--arz") {
             opt.flags |= TMC_FLAG_ARZ;
+        } else if (a == "--angles") {
+            opt.flags |= TMC_FLAG_ANGLES;
+        } else if (a == "--mco") {
+            write_mco_files = true;
+            opt.flags |= TMC_FLAG_ARZ | TMC_FLAG_ANGLES;
         } else if (a == "--no-profiles") {
             opt.flags |= TMC_FLAG_NO_PROFILES;
         } else if (a == "--waves-per-group") {
@@ -124,8 +133,9 @@ int run_command(int argc, char **argv) {
             return 2;
         }
     }
-    if ((opt.flags & TMC_FLAG_ARZ) && (opt.flags & TMC_FLAG_NO_PROFILES)) {
-        std::fprintf(stderr, "tissuemc: --arz and --no-profiles cannot be combined\n");
+    if ((opt.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES)) && (opt.flags & TMC_FLAG_NO_PROFILES)) {
+        std::fprintf(stderr, "tissuemc: --arz, --angles and --mco cannot be combined with "
+                             "--no-profiles\n");
         return 2;
     }
 
@@ -176,6 +186,16 @@ int run_command(int argc, char **argv) {
         if (json.is_open()) {
             if (r) json << ',';
             tmc::write_json(res, json);
+        }
+        if (write_mco_files) {
+            const std::string name = runs[r].output_name[0] ? runs[r].output_name : "tissuemc.mco";
+            std::ofstream mco(name);
+            if (!mco) {
+                std::fprintf(stderr, "tissuemc: cannot write '%s'\n", name.c_str());
+                return 1;
+            }
+            tmc::write_mco(res, mco);
+            std::printf("  wrote %s\n", name.c_str());
         }
     }
     if (json.is_open()) {

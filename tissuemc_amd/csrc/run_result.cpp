@@ -110,8 +110,11 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
     const size_t nz = static_cast<size_t>(spec.nz > 0 ? spec.nz : 0);
     const size_t nr = static_cast<size_t>(spec.nr > 0 ? spec.nr : 0);
     const bool arz = (opts.flags & TMC_FLAG_ARZ) != 0;
+    const bool angles = (opts.flags & TMC_FLAG_ANGLES) != 0;
+    const size_t na = static_cast<size_t>(spec.na > 0 ? spec.na : 0);
     std::vector<uint64_t> rd(B), tt(B), ab(B * L), ev(B), an(B), bs(B), az(B * nz), rr(B * nr),
-        tr(B * nr), a_rz(arz ? nz * nr : 0);
+        tr(B * nr), a_rz(arz ? nz * nr : 0), ra(angles ? na : 0), ta(angles ? na : 0),
+        rra(angles ? nr * na : 0), tra(angles ? nr * na : 0);
     tmc_out out{};
     out.rd = rd.data();
     out.tt = tt.data();
@@ -123,6 +126,12 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
     out.rr = rr.data();
     out.tr = tr.data();
     out.arz = arz ? a_rz.data() : nullptr;
+    if (angles) {
+        out.ra = ra.data();
+        out.ta = ta.data();
+        out.rra = rra.data();
+        out.tra = tra.data();
+    }
     const int code = tmc_hip_run(&spec, &opts, &out);
     if (code != TMC_OK) rethrow(code);
 
@@ -177,6 +186,17 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
     if (arz) {
         r.a_rz.resize(nz * nr);
         for (size_t i = 0; i < nz * nr; ++i) r.a_rz[i] = static_cast<double>(a_rz[i]) * kUnit;
+    }
+    auto scaled = [](const std::vector<uint64_t> &v) {
+        std::vector<double> d(v.size());
+        for (size_t i = 0; i < v.size(); ++i) d[i] = static_cast<double>(v[i]) * kUnit;
+        return d;
+    };
+    if (angles) {
+        r.rd_a = scaled(ra);
+        r.t_a = scaled(ta);
+        r.rd_ra = scaled(rra);
+        r.t_ra = scaled(tra);
     }
     return r;
 }
@@ -431,7 +451,160 @@ void write_json(const RunResult &run, std::ostream &out) {
         j.key("a_rz");
         j.array(run.a_rz);
     }
+    if (!run.rd_a.empty()) {
+        const std::pair<const char *, const std::vector<double> *> angle[] = {
+            {"rd_a", &run.rd_a}, {"t_a", &run.t_a}, {"rd_ra", &run.rd_ra}, {"t_ra", &run.t_ra}};
+        for (const auto &kv : angle) {
+            j.raw(",");
+            j.key(kv.first);
+            j.array(*kv.second);
+        }
+    }
     j.raw("}");
+}
+
+// ------------------------------------------------------------------ MCML .mco
+
+namespace {
+
+void put(std::ostream &o, const char *fmt, double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, fmt, v);
+    o << buf;
+}
+
+// One value per line, as MCML writes its 1-D arrays.
+void column(std::ostream &o, const char *header, const std::vector<double> &v) {
+    o << header << "\n";
+    for (double x : v) {
+        put(o, "%12.4E", x);
+        o << "\n";
+    }
+    o << "\n";
+}
+
+// Five values per line, as MCML writes its 2-D arrays.
+void rows_of_five(std::ostream &o, const std::vector<double> &v) {
+    for (size_t i = 0; i < v.size(); ++i) {
+        put(o, "%12.4E ", v[i]);
+        if (i % 5 == 4) o << "\n";
+    }
+    o << "\n\n";
+}
+
+}  // namespace
+
+void write_mco(const RunResult &run, std::ostream &o) {
+    const tmc_spec &s = run.spec;
+    const size_t nz = static_cast<size_t>(s.nz), nr = static_cast<size_t>(s.nr),
+                 na = static_cast<size_t>(s.na), L = static_cast<size_t>(s.num_layers);
+    if (run.a_rz.size() != nz * nr || run.rd_a.size() != na || run.rd_ra.size() != nr * na)
+        throw ConfigError("an .mco file needs the A(r,z) and exit-angle profiles (--arz --angles)");
+    const double pi = 3.14159265358979323846;
+    const double N = static_cast<double>(run.photons());
+    const double dz = s.dz, dr = s.dr, da = pi / (2.0 * static_cast<double>(na));
+    // ring area 2 pi (ir + 1/2) dr^2 (exact for an annulus) and solid angle of a cone shell
+    // 2 pi (cos a - cos(a + da)) = 4 pi sin((ia + 1/2) da) sin(da / 2)
+    auto area = [&](size_t ir) { return 2.0 * pi * (static_cast<double>(ir) + 0.5) * dr * dr; };
+    auto omega = [&](size_t ia) {
+        return 4.0 * pi * std::sin((static_cast<double>(ia) + 0.5) * da) * std::sin(0.5 * da);
+    };
+
+    char buf[160];
+    o << "A1 \t# Version number of the file format.\n\n"
+         "####\n# Data categories include: \n# InParm, RAT, \n"
+         "# A_l, A_z, Rd_r, Rd_a, Tt_r, Tt_a, \n# A_rz, Rd_ra, Tt_ra \n####\n\n";
+    std::snprintf(buf, sizeof buf, "# User time: %8.2G sec = %8.2G hr.  Simulation time of this run.\n\n",
+                  run.wall_seconds, run.wall_seconds / 3600.0);
+    o << buf;
+    o << "InParm \t\t\t# Input parameters. cm is used.\n";
+    o << (s.output_name[0] ? s.output_name : "tissuemc.mco") << " \tA\t\t# output file name, ASCII.\n";
+    o << static_cast<unsigned long long>(s.photons) << " \t\t\t# No. of photons\n";
+    std::snprintf(buf, sizeof buf, "%G\t%G\t\t# dz, dr [cm]\n", s.dz, s.dr);
+    o << buf;
+    std::snprintf(buf, sizeof buf, "%d\t%d\t%d\t# No. of dz, dr, da.\n\n", s.nz, s.nr, s.na);
+    o << buf;
+    std::snprintf(buf, sizeof buf, "%d\t\t\t\t\t# Number of layers\n", s.num_layers);
+    o << buf << "#n\tmua\tmus\tg\td\t# One line for each layer\n";
+    std::snprintf(buf, sizeof buf, "%G\t\t\t\t\t# n for medium above\n", s.n_above);
+    o << buf;
+    for (size_t l = 0; l < L; ++l) {
+        const tmc_layer &y = s.layers[l];
+        std::snprintf(buf, sizeof buf, "%G\t%G\t%G\t%G\t%G\t# layer %zu\n", y.n, y.mua, y.mus, y.g,
+                      y.d, l + 1);
+        o << buf;
+    }
+    std::snprintf(buf, sizeof buf, "%G\t\t\t\t\t# n for medium below\n\n", s.n_below);
+    o << buf;
+
+    double rd = 0, tt = 0;
+    std::vector<double> a_l(L, 0.0);
+    for (const BatchResult &b : run.batches) {
+        rd += b.rd;
+        tt += b.tt;
+        for (size_t l = 0; l < L && l < b.absorbed.size(); ++l) a_l[l] += b.absorbed[l];
+    }
+    double a = 0;
+    for (double &v : a_l) {
+        a += v;
+        v /= N;
+    }
+    o << "RAT #Reflectance, absorption, transmission. \n";
+    put(o, "%-14.6G", run.specular);
+    o << " \t#Specular reflectance [-]\n";
+    put(o, "%-14.6G", rd / N);
+    o << " \t#Diffuse reflectance [-]\n";
+    put(o, "%-14.6G", a / N);
+    o << " \t#Absorbed fraction [-]\n";
+    put(o, "%-14.6G", tt / N);
+    o << " \t#Transmittance [-]\n\n";
+
+    o << "A_l #Absorption as a function of layer. [-]\n";
+    for (double v : a_l) {
+        put(o, "%12.4G", v);
+        o << "\n";
+    }
+    o << "\n";
+
+    std::vector<double> v;
+    v.assign(nz, 0.0);
+    for (size_t i = 0; i < nz; ++i) v[i] = run.a_z[i] / (N * dz);
+    column(o, "A_z #A[0], [1],..A[nz-1]. [1/cm]", v);
+    v.assign(nr, 0.0);
+    for (size_t i = 0; i < nr; ++i) v[i] = run.rd_r[i] / (N * area(i));
+    column(o, "Rd_r #Rd[0], [1],..Rd[nr-1]. [1/cm2]", v);
+    v.assign(na, 0.0);
+    for (size_t i = 0; i < na; ++i) v[i] = run.rd_a[i] / (N * omega(i));
+    column(o, "Rd_a #Rd[0], [1],..Rd[na-1]. [sr-1]", v);
+    v.assign(nr, 0.0);
+    for (size_t i = 0; i < nr; ++i) v[i] = run.t_r[i] / (N * area(i));
+    column(o, "Tt_r #Tt[0], [1],..Tt[nr-1]. [1/cm2]", v);
+    v.assign(na, 0.0);
+    for (size_t i = 0; i < na; ++i) v[i] = run.t_a[i] / (N * omega(i));
+    column(o, "Tt_a #Tt[0], [1],..Tt[na-1]. [sr-1]", v);
+
+    o << "# A[r][z]. [1/cm3]\n# A[0][0], [0][1],..[0][nz-1]\n# A[1][0], [1][1],..[1][nz-1]\n"
+         "# ...\n# A[nr-1][0], [nr-1][1],..[nr-1][nz-1]\nA_rz\n";
+    v.assign(nr * nz, 0.0);
+    for (size_t ir = 0; ir < nr; ++ir)
+        for (size_t iz = 0; iz < nz; ++iz)
+            v[ir * nz + iz] = run.a_rz[iz * nr + ir] / (N * area(ir) * dz);
+    rows_of_five(o, v);
+    for (int t = 0; t < 2; ++t) {
+        const std::vector<double> &src = t == 0 ? run.rd_ra : run.t_ra;
+        const char *q = t == 0 ? "Rd" : "Tt";
+        std::snprintf(buf, sizeof buf,
+                      "# %s[r][angle]. [1/(cm2sr)].\n# %s[0][0], [0][1],..[0][na-1]\n"
+                      "# %s[1][0], [1][1],..[1][na-1]\n# ...\n"
+                      "# %s[nr-1][0], [nr-1][1],..[nr-1][na-1]\n%s_ra\n",
+                      q, q, q, q, q);
+        o << buf;
+        v.assign(nr * na, 0.0);
+        for (size_t ir = 0; ir < nr; ++ir)
+            for (size_t ia = 0; ia < na; ++ia)
+                v[ir * na + ia] = src[ir * na + ia] / (N * area(ir) * omega(ia));
+        rows_of_five(o, v);
+    }
 }
 
 }  // namespace tmc

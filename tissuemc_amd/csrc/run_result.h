@@ -2,8 +2,9 @@
 //
 // RunResult keeps the reference's fields and meanings (include/tissuemc/result.hpp:15-59) so
 // the JSON a run writes is a superset of the reference's schema (src/io/json_out.cpp:164-306).
-// Additions: T_r, A(r,z), per-bin standard errors of the profiles, boundary-step counts and the
-// launch shape. The GPU's integer tallies (units of 2^-32) are summed as integers, then scaled.
+// Additions: T_r, A(r,z), the exit-angle profiles, per-bin standard errors of the profiles,
+// boundary-step counts and the launch shape. The GPU's integer tallies (units of 2^-32) are
+// summed as integers, then scaled.
 #pragma once
 
 #include <cstdint>
@@ -47,6 +48,8 @@ struct RunResult {
     std::vector<double> a_z, rd_r, t_r;           // summed over batches
     std::vector<double> a_z_se, rd_r_se, t_r_se;  // per-bin standard error of the per-photon mean
     std::vector<double> a_rz;                     // [nz*nr] or empty
+    std::vector<double> rd_a, t_a;                // [na] exit-angle profiles, or empty
+    std::vector<double> rd_ra, t_ra;              // [nr*na], row-major in radius, or empty
     double wall_seconds = 0, kernel_seconds = 0;
     std::string device;
     std::string tally_precision = "int64 fixed point (2^-32)";
@@ -74,5 +77,10 @@ RunResult run_hip(const tmc_spec &spec, const tmc_opts &opts);
 std::vector<tmc_spec> parse_mci_file(const std::string &path);
 
 void write_json(const RunResult &run, std::ostream &out);
+
+// MCML's ASCII output (.mco, "A1" format): the echoed input, then RAT, A_l, A_z, Rd_r, Rd_a,
+// Tt_r, Tt_a, A_rz, Rd_ra, Tt_ra with MCML's normalizations (per photon; per cm of depth, cm^2 of
+// ring area, sr of solid angle). Needs the A(r,z) and angle profiles; throws ConfigError without.
+void write_mco(const RunResult &run, std::ostream &out);
 
 }  // namespace tmc

@@ -124,8 +124,10 @@ struct KernelArgs {
     uint64_t *next;       // [U] ids claimed so far in each unit
     uint64_t *cursor;     // [1] units handed out so far
     uint64_t *rd, *tt, *absorbed, *events, *anomalies, *bsteps, *az, *rr, *tr, *arz;
+    uint64_t *ra, *ta, *rra, *tra;  // exit-angle profiles [na], [nr x na]; na == 0: none
     uint32_t n_units, stack_stride;
-    int32_t lmax, abs_stride, nz, nr;
+    int32_t lmax, abs_stride, nz, nr, na;
+    float inv_da;  // 2 na / pi
     int32_t slot_words, wave_words;    // u64 words: the slot, and slot + layer table, per wave
                                        // (the table starts at the even word after the slot)
     int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] u32 at word tile_off
@@ -291,6 +293,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
     const int off_az = kSlotAbs + a.lmax;
     const float last_z = static_cast<float>(a.nz - 1), last_r = static_cast<float>(a.nr - 1);
+    const float last_a = static_cast<float>(a.na - 1);
     const uint64_t seed = a.key[0];
     const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
 
@@ -466,6 +469,17 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                 uint64_t *row = (exit_top ? a.rr : a.tr) +
                                                 static_cast<uint64_t>(wu.prow) * a.nr;
                                 hbm_add(row + ir, qw);
+                                if (a.na > 0) {
+                                    // MCML's exit angle: from the normal, acos of the
+                                    // escaping |uz| (= ca2), in na bins of pi / (2 na)
+                                    const int ia = bin_of(acosf(ca2), a.inv_da, last_a);
+                                    hbm_add((exit_top ? a.ra : a.ta) + ia, qw);
+                                    hbm_add((exit_top ? a.rra : a.tra) +
+                                                __umul24(static_cast<uint32_t>(ir),
+                                                         static_cast<uint32_t>(a.na)) +
+                                                ia,
+                                            qw);
+                                }
                             }
                         }
                         died = true;
@@ -727,6 +741,7 @@ struct Problem {
     int lmax = 1, abs_stride = 1, nz = 1, nr = 1;
     size_t n_srows = 0, n_prows = 0;
     bool profiles = true, arz = false;
+    int na = 0;  // exit-angle bins tallied (0: none)
     uint64_t photons = 0;
     int reach_z = 1;  // depth bins that can receive weight (for the A(r,z) tile)
 };
@@ -800,7 +815,8 @@ class Resident {
         key_ = alloc(8);
         out_words_ = 1 /*cursor*/ + U /*next*/ + 5 * S + S * p.abs_stride +
                      (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
-                     (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0);
+                     (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
+                     (p.na > 0 ? 2 * static_cast<size_t>(p.na) * (1 + p.nr) : 0);
         out_ = alloc(out_words_ * 8);
         uint64_t *q = static_cast<uint64_t *>(out_);
         a_.cursor = q; q += 1;
@@ -816,7 +832,18 @@ class Resident {
             a_.rr = q; q += PR * p.nr;
             a_.tr = q; q += PR * p.nr;
         }
-        if (p.arz) a_.arz = q;
+        if (p.arz) {
+            a_.arz = q;
+            q += static_cast<size_t>(p.nz) * p.nr;
+        }
+        a_.na = p.na;
+        a_.inv_da = p.na > 0 ? static_cast<float>(2.0 * p.na / M_PI) : 0.0f;
+        if (p.na > 0) {
+            a_.ra = q; q += p.na;
+            a_.ta = q; q += p.na;
+            a_.rra = q; q += static_cast<size_t>(p.nr) * p.na;
+            a_.tra = q;
+        }
         a_.units = static_cast<const UnitRec *>(units_);
         a_.stacks = static_cast<const float *>(stacks_);
         a_.key = static_cast<const uint64_t *>(key_);
@@ -882,7 +909,8 @@ class Resident {
     // Copy results to caller buffers (any may be null).
     void download(uint64_t *rd, uint64_t *tt, uint64_t *absorbed, uint64_t *events,
                   uint64_t *anomalies, uint64_t *bsteps, uint64_t *az, uint64_t *rr, uint64_t *tr,
-                  uint64_t *arz) {
+                  uint64_t *arz, uint64_t *ra = nullptr, uint64_t *ta = nullptr,
+                  uint64_t *rra = nullptr, uint64_t *tra = nullptr) {
         auto d2h = [&](uint64_t *dst, const uint64_t *src, size_t n) {
             if (dst && src && n) HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream_));
         };
@@ -903,6 +931,12 @@ class Resident {
             if (tr) std::memset(tr, 0, PR * a_.nr * 8);
         }
         if (arz_) d2h(arz, a_.arz, static_cast<size_t>(a_.nz) * a_.nr);
+        if (a_.na > 0) {
+            d2h(ra, a_.ra, a_.na);
+            d2h(ta, a_.ta, a_.na);
+            d2h(rra, a_.rra, static_cast<size_t>(a_.nr) * a_.na);
+            d2h(tra, a_.tra, static_cast<size_t>(a_.nr) * a_.na);
+        }
         HIP_TRY(hipStreamSynchronize(stream_));
     }
 
@@ -971,6 +1005,12 @@ tmc_spec checked_single(const tmc_spec &spec_in, const tmc_opts &opt) {
     const bool profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
     if ((opt.flags & TMC_FLAG_ARZ) != 0 && !profiles)
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) is a profile: it cannot be combined with no-profiles");
+    if ((opt.flags & TMC_FLAG_ANGLES) != 0) {
+        if (!profiles)
+            throw EngineError(TMC_ERR_CONFIG,
+                              "the angle profiles cannot be combined with no-profiles");
+        if (spec.na < 1) throw EngineError(TMC_ERR_CONFIG, "the angle profiles need na >= 1");
+    }
     return spec;
 }
 
@@ -985,6 +1025,7 @@ Problem single_problem(const tmc_spec &spec, const tmc_opts &opt) {
     p.nr = spec.nr;
     p.profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
     p.arz = (opt.flags & TMC_FLAG_ARZ) != 0;
+    p.na = (opt.flags & TMC_FLAG_ANGLES) != 0 ? spec.na : 0;
     p.photons = spec.photons;
     p.n_srows = B;
     p.n_prows = B;
@@ -1028,6 +1069,9 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     const bool want_arz = (opt.flags & TMC_FLAG_ARZ) != 0;
     if (want_arz && out->arz == nullptr)
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
+    const bool want_angles = (opt.flags & TMC_FLAG_ANGLES) != 0;
+    if (want_angles && (!out->ra || !out->ta || !out->rra || !out->tra))
+        throw EngineError(TMC_ERR_CONFIG, "angle profiles requested but tmc_out.ra/ta/rra/tra is NULL");
     const DeviceInfo dev = device_info(opt.device);
 
     const Problem p = single_problem(spec, opt);
@@ -1035,7 +1079,9 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     r.upload(p, opt.seed);
     out->kernel_seconds = r.run(false);
     r.download(out->rd, out->tt, out->absorbed, out->events, out->anomalies, out->boundary_steps,
-               out->az, out->rr, out->tr, want_arz ? out->arz : nullptr);
+               out->az, out->rr, out->tr, want_arz ? out->arz : nullptr,
+               want_angles ? out->ra : nullptr, want_angles ? out->ta : nullptr,
+               want_angles ? out->rra : nullptr, want_angles ? out->tra : nullptr);
     {
         const double n0 = spec.n_above, n1 = spec.layers[0].n;
         const double rr = (n0 - n1) / (n0 + n1);
@@ -1120,6 +1166,9 @@ BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &o) {
         throw EngineError(TMC_ERR_CONFIG, "dz, dr must be > 0 and nz, nr >= 1");
     if (static_cast<uint64_t>(n_stacks) * o.batches_per_stack > 0xffffffffull)
         throw EngineError(TMC_ERR_CONFIG, "too many stacks x batches");
+    if ((o.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES)) != 0)
+        throw EngineError(TMC_ERR_CONFIG, "batched runs tally A_z, R_r and T_r per stack; A(r,z) "
+                                          "and the angle profiles are single-run options");
     auto plan = std::make_unique<BatchedPlan>();
     plan->dev = device_info(o.device);
     plan->opt = o;

@@ -18,7 +18,7 @@ MAX_LAYERS = 16
 UNIT = 2.0 ** -32  # weight of one tally unit
 
 TMC_OK, TMC_ERR_CONFIG, TMC_ERR_DEVICE, TMC_ERR_RUNTIME = 0, 1, 2, 3
-FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH = 1, 2, 4
+FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH, FLAG_ANGLES = 1, 2, 4, 8
 
 
 class ConfigError(ValueError):
@@ -58,7 +58,8 @@ class TmcOut(C.Structure):
                 ("kernel_seconds", C.c_double), ("wall_seconds", C.c_double),
                 ("device", C.c_char * 96), ("groups", C.c_int32), ("waves_per_group", C.c_int32),
                 ("lds_bytes", C.c_int32), ("chunk", C.c_int32), ("arz_tile_z", C.c_int32),
-                ("arz_tile_r", C.c_int32)]
+                ("arz_tile_r", C.c_int32), ("ra", _PU64), ("ta", _PU64), ("rra", _PU64),
+                ("tra", _PU64)]
 
 
 class TmcStack(C.Structure):
@@ -242,11 +243,12 @@ def batch_estimate(sums, photons) -> tuple[float, float]:
 
 
 def _opts(*, seed, batches, device, photons, stream, arz, profiles, waves_per_group=0, groups=0,
-          chunk=0, arz_tile=None, step_limit=0) -> TmcOpts:
+          chunk=0, arz_tile=None, step_limit=0, angles=False) -> TmcOpts:
     o = TmcOpts()
     load().tmc_default_opts(C.byref(o))
     o.seed, o.batches, o.device, o.photons, o.stream = seed, int(batches), device, photons, stream
-    o.flags = (FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
+    o.flags = ((FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
+               | (FLAG_ANGLES if angles else 0))
     o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
     if arz_tile is not None:
         o.arz_tile_z, o.arz_tile_r = arz_tile
@@ -286,11 +288,12 @@ def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, 
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
         arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
         groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
-        step_limit: int = 0) -> dict:
+        step_limit: int = 0, angles: bool = False) -> dict:
     """Simulate one run on the GPU; returns the reference's result schema plus extras.
 
-    Extras: ``t_r``, ``a_rz`` (if ``arz``), ``profile_se`` (per-bin standard errors), per-batch
-    profiles ``az_b``/``rr_b``/``tr_b`` and the raw integer tallies under ``fixed``.
+    Extras: ``t_r``, ``a_rz`` (if ``arz``), the exit-angle profiles ``rd_a``/``t_a`` [na] and
+    ``rd_ra``/``t_ra`` [nr, na] (if ``angles``), ``profile_se`` (per-bin standard errors),
+    per-batch profiles ``az_b``/``rr_b``/``tr_b`` and the raw integer tallies under ``fixed``.
     """
     L = load()
     cs = spec_to_c(spec)
@@ -298,18 +301,23 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     B, NL, nz, nr = int(batches), cs.num_layers, cs.nz, cs.nr
     o = _opts(seed=seed, batches=B, device=device, photons=n_ph, stream=stream, arz=arz,
               profiles=profiles, waves_per_group=waves_per_group, groups=groups, chunk=chunk,
-              arz_tile=arz_tile, step_limit=step_limit)
+              arz_tile=arz_tile, step_limit=step_limit, angles=angles)
     if B < 1:
         raise ConfigError("batches must be >= 1")
+    na = max(int(cs.na), 1)
     rd, tt, ev, an, bs = (np.zeros(B, np.uint64) for _ in range(5))
     ab = np.zeros(B * NL, np.uint64)
     az = np.zeros(B * nz, np.uint64)
     rr = np.zeros(B * nr, np.uint64)
     tr = np.zeros(B * nr, np.uint64)
     a_rz = np.zeros(nz * nr, np.uint64) if arz else None
+    ang = [np.zeros(na, np.uint64), np.zeros(na, np.uint64), np.zeros(nr * na, np.uint64),
+           np.zeros(nr * na, np.uint64)] if angles else None
     out = TmcOut(rd=_p(rd), tt=_p(tt), absorbed=_p(ab), events=_p(ev), anomalies=_p(an),
                  boundary_steps=_p(bs), az=_p(az), rr=_p(rr), tr=_p(tr),
                  arz=_p(a_rz) if arz else None)
+    if angles:
+        out.ra, out.ta, out.rra, out.tra = (_p(x) for x in ang)
     code = L.tmc_hip_run(C.byref(cs), C.byref(o), C.byref(out))
     if code != TMC_OK:
         _raise(code)
@@ -370,6 +378,13 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     res["summary"] = summ
     if arz:
         res["a_rz"] = (a_rz.astype(np.float64) * UNIT).reshape(nz, nr)
+    if angles:
+        res["fixed"].update(ra=ang[0], ta=ang[1], rra=ang[2].reshape(nr, na),
+                            tra=ang[3].reshape(nr, na))
+        res["rd_a"] = ang[0].astype(np.float64) * UNIT
+        res["t_a"] = ang[1].astype(np.float64) * UNIT
+        res["rd_ra"] = (ang[2].astype(np.float64) * UNIT).reshape(nr, na)
+        res["t_ra"] = (ang[3].astype(np.float64) * UNIT).reshape(nr, na)
     return res
 
 
@@ -438,12 +453,12 @@ class BatchedPlan:
                  profiles: bool = True, graph: bool = True, waves_per_group: int = 0,
                  groups: int = 0, chunk: int = 0, step_limit: int = 0, photons_hint: int = 0):
         self.n_stacks, self.bs, self.nz, self.nr = n_stacks, batches_per_stack, nz, nr
+        self._h = C.c_void_p()
+        flags = (0 if profiles else FLAG_NO_PROFILES) | (0 if graph else FLAG_NO_GRAPH)
         o = TmcBatchedOpts(seed=1, batches_per_stack=batches_per_stack, device=device, dz=dz,
-                           dr=dr, nz=nz, nr=nr,
-                           flags=(0 if profiles else FLAG_NO_PROFILES) | (0 if graph else FLAG_NO_GRAPH),
+                           dr=dr, nz=nz, nr=nr, flags=flags,
                            waves_per_group=waves_per_group, groups=groups, chunk=chunk,
                            step_limit=step_limit, photons_hint=photons_hint)
-        self._h = C.c_void_p()
         code = load().tmc_batched_create(n_stacks, C.byref(o), C.byref(self._h))
         if code != TMC_OK:
             _raise(code)

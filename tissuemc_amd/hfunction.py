@@ -92,3 +92,26 @@ def semi_infinite_isotropic(mua: float, mus: float, n_medium: float, n_ambient: 
     escape = float(np.sum(wt * k1 * ((1.0 - R) + R * escape_from)))
     specular = ((n_medium - n_ambient) / (n_medium + n_ambient)) ** 2
     return SemiInfinite(specular=specular, diffuse=(1.0 - specular) * escape)
+
+
+def escape_angle_fractions(mua: float, mus: float, na: int, n_quad: int = 32,
+                           n_nodes: int = 200) -> np.ndarray:
+    """Fraction of a normally incident pencil beam that leaves a matched (n = 1) semi-infinite
+    isotropic medium at an angle from the normal in each of na bins of pi / (2 na): the integral
+    of K(mu, 1) = w mu H(mu) H(1) / (2 (mu + 1)) over cos of the bin's angles. The bins sum to the
+    diffuse reflectance."""
+    if mua <= 0 or mus < 0:
+        raise ValueError("need mua > 0 and mus >= 0")
+    albedo = mus / (mua + mus)
+    mu, wt, h = h_nodes(albedo, n_nodes)
+    h1 = h_value(1.0, albedo, mu, wt, h)
+    da = np.pi / (2.0 * na)
+    x, wq = nodes_01(n_quad)
+    out = np.zeros(na)
+    for i in range(na):
+        lo, hi = np.cos((i + 1) * da), np.cos(i * da)
+        m = lo + (hi - lo) * x
+        hm = np.array([h_value(float(v), albedo, mu, wt, h) for v in m])
+        out[i] = (hi - lo) * float(np.sum(wq * albedo * m * hm * h1 / (2.0 * (m + 1.0))))
+    return out
+
