@@ -141,25 +141,9 @@ def bench_surrogate(mc_photons_per_s: float | None = None) -> dict:
     of training steps (4 heads, 30 -> 4 x 512 GELU -> 202, batch 4096), device time by HIP
     events; errors and the matched-error comparison come from the full pipeline run recorded in
     profiles/surrogate_pipeline_full.json."""
-    import numpy as np
+    from tissuemc_amd.mlp import bench_kernels
 
-    from tissuemc_amd.mlp import Mlp
-
-    rng = np.random.default_rng(0)
-    x = rng.uniform(-1, 1, (1_050_000, 30)).astype(np.float32)
-    with Mlp() as net:
-        net.predict(x[:20000])  # warm-up
-        _, _, _, sec = net.predict(x)
-        per_step, per_query = net.flops()
-        net.set_data(x[:65536], rng.normal(0, 1, (65536, 202)).astype(np.float32))
-        net.train(20)
-        _, tsec = net.train(200)
-    out = {"inference_queries": len(x), "inference_device_s": sec,
-           "inference_stacks_per_s": len(x) / sec,
-           "inference_tflops": per_query * len(x) / sec / 1e12,
-           "inference_mfma_frac": per_query * len(x) / sec / 2.5e15,
-           "train_step_ms": tsec / 200 * 1e3, "train_tflops": per_step * 200 / tsec / 1e12,
-           "train_mfma_frac": per_step * 200 / tsec / 2.5e15}
+    out = bench_kernels(1_050_000, 200)
     rep = ROOT / "profiles" / "surrogate_pipeline_full.json"
     if rep.exists():
         r = json.loads(rep.read_text())

@@ -206,55 +206,102 @@ __global__ void reduce_splits(const float *part, int splits, long long count, lo
     }
 }
 
-// Batch assembly: head h, row m takes dataset sample idx (Philox of (step, m, h) mod n), writes
-// the standardized features padded to in_pad as bf16 X [B x in_pad] and Xt [in_pad x B], and the
-// targets padded to out_pad as fp32 Y [B x out_pad].
-__global__ void gather_batch(const float *xd, const float *yd, int64_t n, int in, int out, int in_pad,
-                             int out_pad, int B, const uint64_t *step, uint32_t key0, uint32_t key1,
-                             bf16 *X, bf16 *Xt, float *Y, const int64_t *fixed_rows) {
-    const int m = static_cast<int>(blockIdx.x);
+// Batch assembly for a block of 64 rows of head h: row m takes dataset sample idx (Philox of
+// (step, m, h) mod n); writes the standardized features padded to in_pad as bf16 X [B x in_pad]
+// and, through an LDS tile, Xt [in_pad x B] (both coalesced), and the targets padded to out_pad
+// as fp32 Y [B x out_pad].
+__global__ __launch_bounds__(256) void gather_batch(const float *xd, const float *yd, int64_t n,
+                                                    int in, int out, int in_pad, int out_pad, int B,
+                                                    const uint64_t *step, uint32_t key0,
+                                                    uint32_t key1, bf16 *X, bf16 *Xt, float *Y,
+                                                    const int64_t *fixed_rows) {
+    __shared__ int64_t rows[64];
+    __shared__ bf16 tile[64][66];  // [column][row], padded against bank conflicts
+    const int tx = static_cast<int>(threadIdx.x & 63u), ty = static_cast<int>(threadIdx.x >> 6);
+    const int m0 = static_cast<int>(blockIdx.x) * 64;
     const int h = static_cast<int>(blockIdx.y);
-    int64_t idx;
-    if (fixed_rows) {
-        idx = fixed_rows[m];
-    } else {
-        const Block4 v = philox4x32_10(static_cast<uint32_t>(m), static_cast<uint32_t>(h),
-                                       static_cast<uint32_t>(step[0]), 0x4D4C5031u, key0, key1);
-        idx = static_cast<int64_t>((static_cast<uint64_t>(v.v0) * static_cast<uint64_t>(n)) >> 32);
+    if (ty == 0) {
+        const int m = m0 + tx;
+        int64_t idx;
+        if (fixed_rows) {
+            idx = fixed_rows[m];
+        } else {
+            const Block4 v = philox4x32_10(static_cast<uint32_t>(m), static_cast<uint32_t>(h),
+                                           static_cast<uint32_t>(step[0]), 0x4D4C5031u, key0, key1);
+            idx = static_cast<int64_t>((static_cast<uint64_t>(v.v0) * static_cast<uint64_t>(n)) >> 32);
+        }
+        rows[tx] = idx;
     }
+    __syncthreads();
     bf16 *Xh = X + static_cast<long long>(h) * B * in_pad;
     bf16 *Xth = Xt + static_cast<long long>(h) * B * in_pad;
     float *Yh = Y + static_cast<long long>(h) * B * out_pad;
-    for (int j = static_cast<int>(threadIdx.x); j < in_pad; j += static_cast<int>(blockDim.x)) {
-        const bf16 v = to_bf16(j < in ? xd[idx * in + j] : 0.0f);
-        Xh[static_cast<long long>(m) * in_pad + j] = v;
-        Xth[static_cast<long long>(j) * B + m] = v;
+    for (int j0 = 0; j0 < in_pad; j0 += 64) {
+        const int j = j0 + tx;
+        for (int i = ty; i < 64; i += 4) {
+            const bf16 v = to_bf16(j < in ? xd[rows[i] * in + j] : 0.0f);
+            Xh[static_cast<long long>(m0 + i) * in_pad + j] = v;
+            tile[tx][i] = v;
+        }
+        __syncthreads();
+        for (int i = ty; i < 64; i += 4) Xth[static_cast<long long>(j0 + i) * B + m0 + tx] = tile[i][tx];
+        __syncthreads();
     }
-    for (int j = static_cast<int>(threadIdx.x); j < out_pad; j += static_cast<int>(blockDim.x))
-        Yh[static_cast<long long>(m) * out_pad + j] = j < out ? yd[idx * out + j] : 0.0f;
+    for (int i = ty; i < 64; i += 4)
+        for (int j = tx; j < out_pad; j += 64)
+            Yh[static_cast<long long>(m0 + i) * out_pad + j] = j < out ? yd[rows[i] * out + j] : 0.0f;
 }
 
-// Weighted mean squared error over the real outputs; writes dY (bf16, both orientations) and
-// adds the batch loss of each head into loss[(step % ring) * heads + h].
-__global__ void loss_grad(const float *Yp, const float *Yt, const float *w, int B, int out_pad,
-                          float inv_count, bf16 *dY, bf16 *dYt, float *loss, const uint64_t *step,
-                          int heads, uint64_t ring) {
-    const int m = static_cast<int>(blockIdx.x);
-    const int h = static_cast<int>(blockIdx.y);
-    const long long base = (static_cast<long long>(h) * B + m) * out_pad;
-    float part = 0.0f;
-    for (int j = static_cast<int>(threadIdx.x); j < out_pad; j += static_cast<int>(blockDim.x)) {
-        const float d = Yp[base + j] - Yt[base + j];
-        const float wj = w[j];
-        part += wj * d * d;
+// Weighted squared error over the real outputs on a 64-row x 64-column tile of head h: writes dY
+// (bf16) and, through an LDS tile, its transpose dYt (both coalesced), and the tile's share of
+// the loss into part[h][tile] (summed in a fixed order by loss_reduce).
+__global__ __launch_bounds__(256) void loss_grad(const float *Yp, const float *Yt, const float *w,
+                                                 int B, int out_pad, float inv_count, bf16 *dY,
+                                                 bf16 *dYt, float *part) {
+    __shared__ bf16 tile[64][66];
+    __shared__ float red[256];
+    const int tx = static_cast<int>(threadIdx.x & 63u), ty = static_cast<int>(threadIdx.x >> 6);
+    const int m0 = static_cast<int>(blockIdx.x) * 64, j0 = static_cast<int>(blockIdx.y) * 64;
+    const int h = static_cast<int>(blockIdx.z);
+    const long long hb = static_cast<long long>(h) * B;
+    const float wj = w[j0 + tx];
+    float acc = 0.0f;
+    for (int i = ty; i < 64; i += 4) {
+        const long long e = (hb + m0 + i) * out_pad + j0 + tx;
+        const float d = Yp[e] - Yt[e];
+        acc += wj * d * d;
         const bf16 g = to_bf16(2.0f * wj * d * inv_count);
-        dY[base + j] = g;
-        dYt[(static_cast<long long>(h) * out_pad + j) * B + m] = g;
+        dY[e] = g;
+        tile[tx][i] = g;
     }
-    // wave64 reduction, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-    if ((threadIdx.x & 63u) == 0 && loss)
-        atomicAdd(loss + static_cast<long long>(step[0] % ring) * heads + h, part * inv_count);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4)
+        dYt[(static_cast<long long>(h) * out_pad + j0 + i) * B + m0 + tx] = tile[i][tx];
+    for (int off = 128; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[(static_cast<long long>(h) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// The batch loss of each head: its n tile partials summed in a fixed order, written to
+// loss[(step % ring) * heads + h].
+__global__ void loss_reduce(const float *part, int n, float inv_count, float *loss,
+                            const uint64_t *step, int heads, uint64_t ring) {
+    __shared__ float red[256];
+    const int h = static_cast<int>(blockIdx.x);
+    float s = 0.0f;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x))
+        s += part[static_cast<long long>(h) * n + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = static_cast<int>(blockDim.x) / 2; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[static_cast<long long>(step[0] % ring) * heads + h] = red[0] * inv_count;
 }
 
 // Bias gradient: db[h][n] = sum_m dZt[h][n][m] (rows of the transposed gradient are contiguous).
@@ -449,6 +496,7 @@ class MlpModel {
         lr_ = alloc<float>(1);
         lossw_ = alloc<float>(out_pad_);
         loss_ = alloc<float>(kLossRing * H_);
+        lpart_ = alloc<float>(static_cast<long long>(H_) * (B_ / 64) * (out_pad_ / 64));
         const long long BH = static_cast<long long>(B_) * hid_;
         X_ = alloc<bf16>(H_ * B_ * in_pad_);
         Xt_ = alloc<bf16>(H_ * B_ * in_pad_);
@@ -561,7 +609,6 @@ class MlpModel {
         uint64_t step0 = 0;
         MLP_TRY(hipMemcpy(&step0, step_, 8, hipMemcpyDeviceToHost));
         const int64_t slot = static_cast<int64_t>(step0 % kLossRing);
-        MLP_TRY(hipMemsetAsync(loss_ + slot * H_, 0, static_cast<size_t>(H_) * 4, stream_));
         enqueue_step(drows, false);
         MLP_TRY(hipStreamSynchronize(stream_));
         (void)hipFree(drows);
@@ -798,14 +845,17 @@ class MlpModel {
     void enqueue_step(const int64_t *fixed_rows, bool update) {
         const uint32_t k0 = static_cast<uint32_t>(cfg_.seed ^ 0x5DEECE66DULL);
         const uint32_t k1 = static_cast<uint32_t>((cfg_.seed >> 32) ^ 0xB5297A4DULL);
-        hipLaunchKernelGGL(mlp::gather_batch, dim3(B_, H_), dim3(64), 0, stream_, xd_, yd_, n_data_,
+        hipLaunchKernelGGL(mlp::gather_batch, dim3(B_ / 64, H_), dim3(256), 0, stream_, xd_, yd_, n_data_,
                            cfg_.inputs, cfg_.outputs, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
                            Yt_, fixed_rows);
         MLP_TRY(hipGetLastError());
         forward(X_, B_, Hs_.data(), Z_.data(), Ht_.data(), false);
         const float inv_count = 1.0f / (static_cast<float>(B_) * cfg_.outputs);
-        hipLaunchKernelGGL(mlp::loss_grad, dim3(B_, H_), dim3(64), 0, stream_, Yp_, Yt_, lossw_, B_,
-                           out_pad_, inv_count, dY_, dYt_, loss_, step_, H_,
+        hipLaunchKernelGGL(mlp::loss_grad, dim3(B_ / 64, out_pad_ / 64, H_), dim3(256), 0, stream_, Yp_,
+                           Yt_, lossw_, B_, out_pad_, inv_count, dY_, dYt_, lpart_);
+        MLP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(mlp::loss_reduce, dim3(H_), dim3(256), 0, stream_, lpart_,
+                           (B_ / 64) * (out_pad_ / 64), inv_count, loss_, step_, H_,
                            static_cast<uint64_t>(kLossRing));
         MLP_TRY(hipGetLastError());
         const long long BH = static_cast<long long>(B_) * hid_;
@@ -896,6 +946,7 @@ class MlpModel {
     int splits_ = 1;
     float *part_ = nullptr;
     float *p_ = nullptr, *g_ = nullptr, *m_ = nullptr, *v_ = nullptr, *lossw_ = nullptr, *loss_ = nullptr;
+    float *lpart_ = nullptr;  // per-tile loss partials [heads x tiles]
     bf16 *wb_ = nullptr;
     uint64_t *step_ = nullptr;
     float *lr_ = nullptr;

@@ -152,3 +152,43 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+def bench_kernels(queries: int = 1_050_000, train_steps: int = 200, seed: int = 0) -> dict:
+    """The surrogate's kernels at the pipeline's full size with random weights and inputs:
+    ensemble inference on `queries` rows and `train_steps` graph-replayed Adam steps (4 heads,
+    30 -> 4 x 512 GELU -> 202, batch 4096). Device times are HIP events."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (queries, 30)).astype(np.float32)
+    with Mlp() as net:
+        net.predict(x[:20000])  # warm-up
+        _, _, _, sec = net.predict(x)
+        per_step, per_query = net.flops()
+        net.set_data(x[:65536], rng.normal(0, 1, (65536, 202)).astype(np.float32))
+        net.train(20)
+        _, tsec = net.train(train_steps)
+    return {"inference_queries": queries, "inference_device_s": sec,
+            "inference_stacks_per_s": queries / sec,
+            "inference_tflops": per_query * queries / sec / 1e12,
+            "inference_mfma_frac": per_query * queries / sec / 2.5e15,
+            "train_step_ms": tsec / train_steps * 1e3,
+            "train_tflops": per_step * train_steps / tsec / 1e12,
+            "train_mfma_frac": per_step * train_steps / tsec / 2.5e15}
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(prog="python -m tissuemc_amd.mlp",
+                                 description="time the surrogate network's kernels")
+    ap.add_argument("command", choices=["bench"])
+    ap.add_argument("--queries", type=int, default=1_050_000)
+    ap.add_argument("--train-steps", type=int, default=200)
+    args = ap.parse_args(argv)
+    print(json.dumps(bench_kernels(args.queries, args.train_steps)))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
