@@ -18,7 +18,7 @@
 // MFMA: v_mfma_f32_16x16x32_bf16. Lane l holds A row l%16 and B column l%16, k = 8*(l/16)..+7;
 // the 16x16 result has column l%16, rows 4*(l/16)..+3 (checked on the device). A workgroup of
 // four waves computes a 128x128 tile, each wave 64x64 (4x4 MFMA tiles), K in steps of 32, operands
-// loaded straight from global memory as 16-byte vectors (see gemm_nt).
+// staged through double-buffered LDS tiles (see gemm_nt).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,22 +90,45 @@ struct GemmArgs {
 };
 
 // C[m, n] = sum_k A[m, k] B[n, k] on a 128 x 128 tile per workgroup: four waves in a 2 x 2
-// arrangement, each accumulating 64 x 64 as 4 x 4 MFMA tiles of 16 x 16. Each K step of 32 needs
-// four A and four B fragments per lane (16-byte loads); the next step's fragments are loaded
-// before the current step's 16 MFMAs are issued.
+// arrangement, each accumulating 64 x 64 as 4 x 4 MFMA tiles of 16 x 16.
+//
+// K advances in stages of 64. Each stage's A and B tiles (128 x 64 bf16 each) are read from
+// global memory once per workgroup, coalesced (eight threads per 128-byte row segment), into
+// registers while the previous stage is being multiplied, then stored into the other half of a
+// double-buffered LDS tile: one barrier per stage. Each wave then reads its MFMA fragments from
+// LDS (16 bytes per lane). Rows are 128 bytes, so the 16-byte chunks of row r are stored at chunk
+// c ^ f(r), f(r) = (r & 7) ^ ((r >> 3) & 1): the 16 rows of a fragment land in distinct banks
+// whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle.
+constexpr int kBK = 64;                       // K per stage
+constexpr int kTileElems = 128 * kBK;         // bf16 elements of one 128 x 64 tile
+
+__device__ __forceinline__ int swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    const int wave = static_cast<int>(threadIdx.x >> 6);
+    __shared__ __attribute__((aligned(16))) bf16 smem[2 * 2 * kTileElems];  // [buf][A|B][128][64]
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int z = static_cast<int>(blockIdx.z);
     const int h = z / g.splits, split = z % g.splits;
-    const int m0 = static_cast<int>(blockIdx.y) * 128 + (wave >> 1) * 64;
-    const int n0 = static_cast<int>(blockIdx.x) * 128 + (wave & 1) * 64;
+    const int mblk = static_cast<int>(blockIdx.y) * 128, nblk = static_cast<int>(blockIdx.x) * 128;
+    const int m0 = mblk + (wave >> 1) * 64;
+    const int n0 = nblk + (wave & 1) * 64;
     const int kspan = g.K / g.splits;
-    const int kbeg = split * kspan, kend = kbeg + kspan;
+    const int kbeg = split * kspan;
+    const int nk = kspan / kBK;
     const bf16 *A = g.A + h * g.sA;
     const bf16 *B = g.B + h * g.sB;
-    const int r = lane & 15, kq = (lane >> 4) * 8;
+
+    // global -> LDS mapping: thread t moves chunk (t % 8) of rows t / 8 + 32 q, q = 0..3
+    const int lrow = tid >> 3, lchunk = tid & 7;
+    const bf16 *ag = A + static_cast<long long>(mblk + lrow) * g.lda + kbeg + lchunk * 8;
+    const bf16 *bg = B + static_cast<long long>(nblk + lrow) * g.ldb + kbeg + lchunk * 8;
+    const int st_off = lrow * kBK + ((lchunk ^ swz(lrow)) * 8);  // + q * 32 rows
+    // fragment mapping: lane holds row (lane % 16), k chunk (lane / 16) of a 32-wide step
+    const int r = lane & 15, kq = lane >> 4, fr = swz(r);
+    const int arow = ((wave >> 1) * 64 + r) * kBK, brow = ((wave & 1) * 64 + r) * kBK;
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -113,37 +136,59 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    const bf16 *ap[4], *bp[4];
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;  // one stage in flight, in registers
+#define TMC_GEMM_LOAD(kt)                                                                         \
+    do {                                                                                          \
+        const long long ko = static_cast<long long>(kt) * kBK;                                   \
+        ra0 = *reinterpret_cast<const uint4 *>(ag + ko);                                          \
+        ra1 = *reinterpret_cast<const uint4 *>(ag + 32LL * g.lda + ko);                           \
+        ra2 = *reinterpret_cast<const uint4 *>(ag + 64LL * g.lda + ko);                           \
+        ra3 = *reinterpret_cast<const uint4 *>(ag + 96LL * g.lda + ko);                           \
+        rb0 = *reinterpret_cast<const uint4 *>(bg + ko);                                          \
+        rb1 = *reinterpret_cast<const uint4 *>(bg + 32LL * g.ldb + ko);                           \
+        rb2 = *reinterpret_cast<const uint4 *>(bg + 64LL * g.ldb + ko);                           \
+        rb3 = *reinterpret_cast<const uint4 *>(bg + 96LL * g.ldb + ko);                           \
+    } while (0)
+#define TMC_GEMM_STORE(buf)                                                                       \
+    do {                                                                                          \
+        bf16 *As_ = smem + (buf) * 2 * kTileElems + st_off, *Bs_ = As_ + kTileElems;              \
+        *reinterpret_cast<uint4 *>(As_) = ra0;                                                    \
+        *reinterpret_cast<uint4 *>(As_ + 32 * kBK) = ra1;                                         \
+        *reinterpret_cast<uint4 *>(As_ + 64 * kBK) = ra2;                                         \
+        *reinterpret_cast<uint4 *>(As_ + 96 * kBK) = ra3;                                         \
+        *reinterpret_cast<uint4 *>(Bs_) = rb0;                                                    \
+        *reinterpret_cast<uint4 *>(Bs_ + 32 * kBK) = rb1;                                         \
+        *reinterpret_cast<uint4 *>(Bs_ + 64 * kBK) = rb2;                                         \
+        *reinterpret_cast<uint4 *>(Bs_ + 96 * kBK) = rb3;                                         \
+    } while (0)
+
+    TMC_GEMM_LOAD(0);
+    TMC_GEMM_STORE(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) TMC_GEMM_LOAD(kt + 1);
+        const bf16 *As = smem + buf * 2 * kTileElems, *Bs = As + kTileElems;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        ap[i] = A + static_cast<long long>(m0 + 16 * i + r) * g.lda + kq;
-        bp[i] = B + static_cast<long long>(n0 + 16 * i + r) * g.ldb + kq;
-    }
-    bf16x8 fa[4], fb[4];
+        for (int ks = 0; ks < kBK / 32; ++ks) {
+            const int ch = ((ks * 4 + kq) ^ fr) * 8;
+            bf16x8 fa[4], fb[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        fa[i] = *reinterpret_cast<const bf16x8 *>(ap[i] + kbeg);
-        fb[i] = *reinterpret_cast<const bf16x8 *>(bp[i] + kbeg);
-    }
-    for (int k = kbeg; k < kend; k += 32) {
-        bf16x8 na[4], nb[4];
-        const int kn = k + 32 < kend ? k + 32 : k;  // last step reloads the current (unused)
+            for (int i = 0; i < 4; ++i) {
+                fa[i] = *reinterpret_cast<const bf16x8 *>(As + arow + i * 16 * kBK + ch);
+                fb[i] = *reinterpret_cast<const bf16x8 *>(Bs + brow + i * 16 * kBK + ch);
+            }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            na[i] = *reinterpret_cast<const bf16x8 *>(ap[i] + kn);
-            nb[i] = *reinterpret_cast<const bf16x8 *>(bp[i] + kn);
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            fa[i] = na[i];
-            fb[i] = nb[i];
-        }
+        if (kt + 1 < nk) TMC_GEMM_STORE(buf ^ 1);
+        __syncthreads();
     }
+#undef TMC_GEMM_LOAD
+#undef TMC_GEMM_STORE
 
     // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
 #pragma unroll
@@ -487,7 +532,10 @@ class MlpModel {
         MLP_TRY(hipEventCreate(&e1_));
         p_ = alloc<float>(H_ * P_);
         g_ = alloc<float>(H_ * P_);
+        // split-K of the weight gradients: about 256 batch rows per split, each split a whole
+        // number of 64-row GEMM stages
         splits_ = std::max(1, B_ / 256);
+        while (B_ % (mlp::kBK * splits_) != 0) --splits_;
         part_ = alloc<float>(static_cast<long long>(H_) * splits_ * P_);
         m_ = alloc<float>(H_ * P_);
         v_ = alloc<float>(H_ * P_);
@@ -756,7 +804,7 @@ class MlpModel {
 
     template <int kEpi> void gemm(mlp::GemmArgs g) {
         if (g.splits < 1) g.splits = 1;
-        if (g.M % 128 || g.N % 128 || g.K % (32 * g.splits))
+        if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
             throw EngineError(TMC_ERR_RUNTIME, "gemm_nt shape is not a multiple of the tile");
         hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256), 0,
                            stream_, g);
