@@ -39,10 +39,14 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 typedef uint16_t bf16;  // storage type
 
 __host__ __device__ __forceinline__ bf16 to_bf16(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(bf16, static_cast<__bf16>(f));  // v_cvt_pk_bf16_f32, nearest even
+#else
     uint32_t u;
     std::memcpy(&u, &f, 4);
     u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
     return static_cast<bf16>(u >> 16);
+#endif
 }
 __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
     const uint32_t u = static_cast<uint32_t>(h) << 16;
@@ -53,19 +57,23 @@ __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
 
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluA = 0.044715f;
+constexpr float kLog2E = 1.4426950408889634f;
 
-__device__ __forceinline__ float tanh_fast(float u) {
-    // tanh(u) = 1 - 2 / (exp(2u) + 1), exp through v_exp_f32 (2^x)
-    const float e = __builtin_amdgcn_exp2f(2.0f * 1.4426950408889634f * u);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// GELU in its tanh form, 0.5 z (1 + tanh u) with u = sqrt(2/pi) (z + 0.044715 z^3), evaluated
+// as z s with s = 0.5 (1 + tanh u) = 1 / (1 + e^(-2u)) and e^(-2u) = 2^(z (e1 + e3 z^2)): one
+// v_exp_f32 and one v_rcp_f32.
+constexpr float kGeluE1 = -2.0f * kGeluC * kLog2E;
+constexpr float kGeluE3 = -2.0f * kGeluC * kGeluA * kLog2E;
+__device__ __forceinline__ float gelu_s(float z) {
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(kGeluE3, z * z, kGeluE1));
+    return __builtin_amdgcn_rcpf(1.0f + e);
 }
-__device__ __forceinline__ float gelu(float z) {
-    const float t = tanh_fast(kGeluC * (z + kGeluA * z * z * z));
-    return 0.5f * z * (1.0f + t);
-}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
+// d/dz: s + z (1 - tanh^2 u) u' / 2 = s + 2 z s (1 - s) sqrt(2/pi) (1 + 3 a z^2)
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float t = tanh_fast(kGeluC * (z + kGeluA * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+    const float s = gelu_s(z);
+    const float du = __builtin_fmaf(3.0f * kGeluA * 2.0f * kGeluC, z * z, 2.0f * kGeluC);
+    return __builtin_fmaf(z * s * (1.0f - s), du, s);
 }
 
 enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kBwdWeight = 3 };
@@ -101,12 +109,15 @@ struct GemmArgs {
 // whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle.
 constexpr int kBK = 64;                       // K per stage
 constexpr int kTileElems = 128 * kBK;         // bf16 elements of one 128 x 64 tile
+constexpr int kPitch = 136;                   // bf16 row pitch of the epilogue's output tiles
+constexpr int kGemmLds = 2 * 128 * kPitch * 2;  // bytes: the larger of staging and epilogue use
+static_assert(kGemmLds >= 2 * 2 * kTileElems * 2, "epilogue tiles must cover the staging tiles");
 
 __device__ __forceinline__ int swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
 
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) bf16 smem[2 * 2 * kTileElems];  // [buf][A|B][128][64]
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // [buf][A|B][128][64], then T
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -191,25 +202,22 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 #undef TMC_GEMM_STORE
 
     // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
+    if constexpr (kEpi == kFwdHidden || kEpi == kBwdData) {
+        // bf16 results go through LDS (free after the last stage's barrier): written row-major
+        // into T and, when the transpose is wanted, column-major into Tt, then read back in
+        // 16-byte row pieces, so that every global store is coalesced. The pitch of 136 puts the
+        // four row groups of a fragment 16 banks apart.
+        bf16 *T = smem, *Tt = smem + 128 * kPitch;
+        bf16 *Cb = g.Cb + h * g.sCb;
+        bf16 *Ct = g.Ct ? g.Ct + h * g.sCb : nullptr;
+        const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = n0 + 16 * j + r;
-            const int mb = m0 + 16 * i + (lane >> 4) * 4;
-            if (kEpi == kBwdWeight) {
-                float *C = g.Cf + h * g.sCf + split * g.sSplit;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
-            } else if (kEpi == kFwdOut) {
-                float *C = g.Cf + h * g.sCf;
-                const float b = g.bias[h * g.sBias + n];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q] + b;
-            } else {
-                bf16 *Cb = g.Cb + h * g.sCb;
-                bf16 *Ct = g.Ct ? g.Ct + h * g.sCb : nullptr;
-                bf16 out[4];
+            for (int j = 0; j < 4; ++j) {
+                const int col = wn + 16 * j + r, rb = wm + 16 * i + (lane >> 4) * 4;
+                const int n = nblk + col, mb = mblk + rb;
+                bf16 o[4];
                 if (kEpi == kFwdHidden) {
                     float *Zp = g.Cf ? g.Cf + h * g.sCf : nullptr;
                     const float b = g.bias[h * g.sBias + n];
@@ -217,21 +225,56 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
                     for (int q = 0; q < 4; ++q) {
                         const float zv = acc[i][j][q] + b;
                         if (Zp) Zp[static_cast<long long>(mb + q) * g.N + n] = zv;
-                        out[q] = to_bf16(gelu(zv));
+                        o[q] = to_bf16(gelu(zv));
                     }
                 } else {  // kBwdData
                     const float *Zp = g.Zin + h * g.sZin;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        out[q] = to_bf16(acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]));
+                        o[q] = to_bf16(acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]));
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) Cb[static_cast<long long>(mb + q) * g.N + n] = out[q];
+                for (int q = 0; q < 4; ++q) T[(rb + q) * kPitch + col] = o[q];
                 if (Ct) {
                     uint2 packed;
-                    packed.x = static_cast<uint32_t>(out[0]) | (static_cast<uint32_t>(out[1]) << 16);
-                    packed.y = static_cast<uint32_t>(out[2]) | (static_cast<uint32_t>(out[3]) << 16);
-                    *reinterpret_cast<uint2 *>(Ct + static_cast<long long>(n) * g.M + mb) = packed;
+                    packed.x = static_cast<uint32_t>(o[0]) | (static_cast<uint32_t>(o[1]) << 16);
+                    packed.y = static_cast<uint32_t>(o[2]) | (static_cast<uint32_t>(o[3]) << 16);
+                    *reinterpret_cast<uint2 *>(Tt + col * kPitch + rb) = packed;
+                }
+            }
+        }
+        __syncthreads();
+        // 128 rows x 16 pieces of 8 bf16, eight pieces per thread
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = tid + 256 * k, row = c >> 4, piece = (c & 15) * 8;
+            *reinterpret_cast<uint4 *>(Cb + static_cast<long long>(mblk + row) * g.N + nblk + piece) =
+                *reinterpret_cast<const uint4 *>(T + row * kPitch + piece);
+        }
+        if (Ct) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int c = tid + 256 * k, row = c >> 4, piece = (c & 15) * 8;
+                *reinterpret_cast<uint4 *>(Ct + static_cast<long long>(nblk + row) * g.M + mblk + piece) =
+                    *reinterpret_cast<const uint4 *>(Tt + row * kPitch + piece);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + 16 * j + r;
+                const int mb = m0 + 16 * i + (lane >> 4) * 4;
+                if (kEpi == kBwdWeight) {
+                    float *C = g.Cf + h * g.sCf + split * g.sSplit;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
+                } else {  // kFwdOut
+                    float *C = g.Cf + h * g.sCf;
+                    const float b = g.bias[h * g.sBias + n];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q] + b;
                 }
             }
         }
@@ -501,6 +544,16 @@ class MlpModel {
             throw EngineError(TMC_ERR_DEVICE, "no usable HIP device found");
         if (c.device < 0 || c.device >= count) throw EngineError(TMC_ERR_CONFIG, "no such device");
         MLP_TRY(hipSetDevice(c.device));
+        // the GEMM's LDS (staging tiles, then the epilogue's output tiles) exceeds the 64 KB that a
+        // launch gets without asking
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kFwdHidden>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kFwdOut>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kBwdData>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kBwdWeight>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
         H_ = c.heads;
         in_pad_ = pad128(c.inputs);
         out_pad_ = pad128(c.outputs);
@@ -806,8 +859,8 @@ class MlpModel {
         if (g.splits < 1) g.splits = 1;
         if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
             throw EngineError(TMC_ERR_RUNTIME, "gemm_nt shape is not a multiple of the tile");
-        hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256), 0,
-                           stream_, g);
+        hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256),
+                           mlp::kGemmLds, stream_, g);
         MLP_TRY(hipGetLastError());
     }
 
