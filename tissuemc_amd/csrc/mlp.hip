@@ -123,7 +123,14 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     const int wave = tid >> 6;
     const int z = static_cast<int>(blockIdx.z);
     const int h = z / g.splits, split = z % g.splits;
-    const int mblk = static_cast<int>(blockIdx.y) * 128, nblk = static_cast<int>(blockIdx.x) * 128;
+    // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin in launch order, so
+    // hardware tile L runs on XCD L % 8. Give XCD k the contiguous logical tiles
+    // [k T / 8, (k + 1) T / 8): the column tiles of one row block then run on one XCD, which
+    // reads that block of A from HBM once into its own L2 instead of once per column tile.
+    const int NX = static_cast<int>(gridDim.x), T = NX * static_cast<int>(gridDim.y);
+    int tile = static_cast<int>(blockIdx.x) + NX * static_cast<int>(blockIdx.y);
+    if (T % 8 == 0) tile = (tile % 8) * (T / 8) + tile / 8;
+    const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 128;
     const int m0 = mblk + (wave >> 1) * 64;
     const int n0 = nblk + (wave & 1) * 64;
     const int kspan = g.K / g.splits;
