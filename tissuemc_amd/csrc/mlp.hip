@@ -100,24 +100,41 @@ struct GemmArgs {
 // C[m, n] = sum_k A[m, k] B[n, k] on a 128 x 128 tile per workgroup: four waves in a 2 x 2
 // arrangement, each accumulating 64 x 64 as 4 x 4 MFMA tiles of 16 x 16.
 //
-// K advances in stages of 64. Each stage's A and B tiles (128 x 64 bf16 each) are read from
-// global memory once per workgroup, coalesced (eight threads per 128-byte row segment), into
-// registers while the previous stage is being multiplied, then stored into the other half of a
-// double-buffered LDS tile: one barrier per stage. Each wave then reads its MFMA fragments from
-// LDS (16 bytes per lane). Rows are 128 bytes, so the 16-byte chunks of row r are stored at chunk
-// c ^ f(r), f(r) = (r & 7) ^ ((r >> 3) & 1): the 16 rows of a fragment land in distinct banks
-// whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle.
-constexpr int kBK = 64;                       // K per stage
-constexpr int kTileElems = 128 * kBK;         // bf16 elements of one 128 x 64 tile
-constexpr int kPitch = 136;                   // bf16 row pitch of the epilogue's output tiles
-constexpr int kGemmLds = 2 * 128 * kPitch * 2;  // bytes: the larger of staging and epilogue use
-static_assert(kGemmLds >= 2 * 2 * kTileElems * 2, "epilogue tiles must cover the staging tiles");
+// K advances in stages of 32. A stage's A and B tiles (128 x 32 bf16 each) go from global memory
+// straight into LDS (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's LDS base +
+// 16 x lane, no registers involved), into a ring of kStages stages, so the loads run
+// kStages - 1 stages ahead of the multiplications. One barrier per stage, after the wave has
+// waited for its own loads of that stage (vmcnt); it also tells every wave that the ring slot the
+// next loads overwrite has been read. Each wave then reads its MFMA fragments from LDS (16 bytes
+// per lane). Rows are 64 bytes (four 16-byte chunks), and chunk c of row r is stored at chunk
+// c ^ s(r), s(r) = ((r >> 1) ^ (r >> 2)) & 3: the 16 rows of a fragment land in distinct banks
+// whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle. The loads pick their global
+// chunk to match, since a lane's destination in LDS is fixed by its lane number.
+constexpr int kBK = 32;                          // K per stage
+constexpr int kStages = 4;                       // LDS ring depth
+constexpr int kStageElems = 2 * 128 * kBK;       // bf16 elements of one stage (A then B)
+constexpr int kPitch = 136;                      // bf16 row pitch of the epilogue's output tiles
+constexpr int kGemmLds = 2 * 128 * kPitch * 2;   // bytes: the larger of staging and epilogue use
+static_assert(kGemmLds >= kStages * kStageElems * 2, "epilogue tiles must cover the staging ring");
 
-__device__ __forceinline__ int swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+__device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+typedef __attribute__((address_space(1))) const void global_cvoid;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Wait for this wave's global loads until at most N are in flight, and for its LDS accesses, then
+// meet the other waves. Written as asm so that the compiler neither waits for every load (a
+// fence would) nor moves LDS accesses across it.
+template <int N>
+__device__ __forceinline__ void stage_sync() {
+    if constexpr (N == 8) asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else if constexpr (N == 4) asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
-    extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // [buf][A|B][128][64], then T
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // staging ring, then T and Tt
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -139,14 +156,19 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     const bf16 *A = g.A + h * g.sA;
     const bf16 *B = g.B + h * g.sB;
 
-    // global -> LDS mapping: thread t moves chunk (t % 8) of rows t / 8 + 32 q, q = 0..3
-    const int lrow = tid >> 3, lchunk = tid & 7;
-    const bf16 *ag = A + static_cast<long long>(mblk + lrow) * g.lda + kbeg + lchunk * 8;
-    const bf16 *bg = B + static_cast<long long>(nblk + lrow) * g.ldb + kbeg + lchunk * 8;
-    const int st_off = lrow * kBK + ((lchunk ^ swz(lrow)) * 8);  // + q * 32 rows
-    // fragment mapping: lane holds row (lane % 16), k chunk (lane / 16) of a 32-wide step
-    const int r = lane & 15, kq = lane >> 4, fr = swz(r);
-    const int arow = ((wave >> 1) * 64 + r) * kBK, brow = ((wave & 1) * 64 + r) * kBK;
+    // loads: wave w moves rows 32 w + 16 i + lane / 4 (i = 0, 1) of A and of B; lane's LDS slot
+    // is chunk lane % 4 of its row, so it fetches the chunk stored there
+    const int lrow0 = 32 * wave + (lane >> 2), lrow1 = lrow0 + 16;
+    const int lc0 = ((lane & 3) ^ swz(lrow0)) * 8, lc1 = ((lane & 3) ^ swz(lrow1)) * 8;
+    const bf16 *a0 = A + static_cast<long long>(mblk + lrow0) * g.lda + kbeg + lc0;
+    const bf16 *a1 = A + static_cast<long long>(mblk + lrow1) * g.lda + kbeg + lc1;
+    const bf16 *b0 = B + static_cast<long long>(nblk + lrow0) * g.ldb + kbeg + lc0;
+    const bf16 *b1 = B + static_cast<long long>(nblk + lrow1) * g.ldb + kbeg + lc1;
+    const int wdst = 32 * wave * kBK;  // the wave's first row in a tile (elements)
+    // fragments: lane holds row (lane % 16), k chunk (lane / 16)
+    const int r = lane & 15, kq = lane >> 4;
+    const int ch = (kq ^ swz(r)) * 8;
+    const int arow = ((wave >> 1) * 64 + r) * kBK + ch, brow = ((wave & 1) * 64 + r) * kBK + ch;
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -154,59 +176,41 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;  // one stage in flight, in registers
-#define TMC_GEMM_LOAD(kt)                                                                         \
+#define TMC_GEMM_ISSUE(st)                                                                        \
     do {                                                                                          \
-        const long long ko = static_cast<long long>(kt) * kBK;                                   \
-        ra0 = *reinterpret_cast<const uint4 *>(ag + ko);                                          \
-        ra1 = *reinterpret_cast<const uint4 *>(ag + 32LL * g.lda + ko);                           \
-        ra2 = *reinterpret_cast<const uint4 *>(ag + 64LL * g.lda + ko);                           \
-        ra3 = *reinterpret_cast<const uint4 *>(ag + 96LL * g.lda + ko);                           \
-        rb0 = *reinterpret_cast<const uint4 *>(bg + ko);                                          \
-        rb1 = *reinterpret_cast<const uint4 *>(bg + 32LL * g.ldb + ko);                           \
-        rb2 = *reinterpret_cast<const uint4 *>(bg + 64LL * g.ldb + ko);                           \
-        rb3 = *reinterpret_cast<const uint4 *>(bg + 96LL * g.ldb + ko);                           \
-    } while (0)
-#define TMC_GEMM_STORE(buf)                                                                       \
-    do {                                                                                          \
-        bf16 *As_ = smem + (buf) * 2 * kTileElems + st_off, *Bs_ = As_ + kTileElems;              \
-        *reinterpret_cast<uint4 *>(As_) = ra0;                                                    \
-        *reinterpret_cast<uint4 *>(As_ + 32 * kBK) = ra1;                                         \
-        *reinterpret_cast<uint4 *>(As_ + 64 * kBK) = ra2;                                         \
-        *reinterpret_cast<uint4 *>(As_ + 96 * kBK) = ra3;                                         \
-        *reinterpret_cast<uint4 *>(Bs_) = rb0;                                                    \
-        *reinterpret_cast<uint4 *>(Bs_ + 32 * kBK) = rb1;                                         \
-        *reinterpret_cast<uint4 *>(Bs_ + 64 * kBK) = rb2;                                         \
-        *reinterpret_cast<uint4 *>(Bs_ + 96 * kBK) = rb3;                                         \
+        bf16 *As_ = smem + ((st) % kStages) * kStageElems + wdst, *Bs_ = As_ + 128 * kBK;         \
+        const int ko = (st) * kBK;                                                                \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a0 + ko), (lds_void *)As_, 16, 0, 0);   \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a1 + ko), (lds_void *)(As_ + 16 * kBK), \
+                                         16, 0, 0);                                               \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + ko), (lds_void *)Bs_, 16, 0, 0);   \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b1 + ko), (lds_void *)(Bs_ + 16 * kBK), \
+                                         16, 0, 0);                                               \
     } while (0)
 
-    TMC_GEMM_LOAD(0);
-    TMC_GEMM_STORE(0);
-    __syncthreads();
+    for (int st = 0; st < kStages - 1 && st < nk; ++st) TMC_GEMM_ISSUE(st);
     for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) TMC_GEMM_LOAD(kt + 1);
-        const bf16 *As = smem + buf * 2 * kTileElems, *Bs = As + kTileElems;
+        // stages issued after kt so far: min(kStages - 2, nk - 1 - kt), four loads each
+        const int ahead = nk - 1 - kt;
+        if (ahead >= 2) stage_sync<8>();
+        else if (ahead == 1) stage_sync<4>();
+        else stage_sync<0>();
+        if (kt + kStages - 1 < nk) TMC_GEMM_ISSUE(kt + kStages - 1);
+        const bf16 *As = smem + (kt % kStages) * kStageElems, *Bs = As + 128 * kBK;
+        bf16x8 fa[4], fb[4];
 #pragma unroll
-        for (int ks = 0; ks < kBK / 32; ++ks) {
-            const int ch = ((ks * 4 + kq) ^ fr) * 8;
-            bf16x8 fa[4], fb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i] = *reinterpret_cast<const bf16x8 *>(As + arow + i * 16 * kBK + ch);
-                fb[i] = *reinterpret_cast<const bf16x8 *>(Bs + brow + i * 16 * kBK + ch);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < 4; ++i) {
+            fa[i] = *reinterpret_cast<const bf16x8 *>(As + arow + i * 16 * kBK);
+            fb[i] = *reinterpret_cast<const bf16x8 *>(Bs + brow + i * 16 * kBK);
         }
-        if (kt + 1 < nk) TMC_GEMM_STORE(buf ^ 1);
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
-#undef TMC_GEMM_LOAD
-#undef TMC_GEMM_STORE
+#undef TMC_GEMM_ISSUE
+    stage_sync<0>();  // every wave is done with the ring before the epilogue reuses the LDS
 
     // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
     if constexpr (kEpi == kFwdHidden || kEpi == kBwdData) {
