@@ -305,22 +305,23 @@ __global__ void reduce_splits(const float *part, int splits, long long count, lo
     }
 }
 
-// Batch assembly for a block of 64 rows of head h: row m takes dataset sample idx (Philox of
+// Batch assembly for a block of 16 rows of head h: row m takes dataset sample idx (Philox of
 // (step, m, h) mod n); writes the standardized features padded to in_pad as bf16 X [B x in_pad]
-// and, through an LDS tile, Xt [in_pad x B] (both coalesced), and the targets padded to out_pad
-// as fp32 Y [B x out_pad].
+// and, through an LDS tile, Xt [in_pad x B], and the targets padded to out_pad as fp32
+// Y [B x out_pad]. Consecutive threads take consecutive columns, so every access is coalesced.
+constexpr int kGatherRows = 16;
 __global__ __launch_bounds__(256) void gather_batch(const float *xd, const float *yd, int64_t n,
                                                     int in, int out, int in_pad, int out_pad, int B,
                                                     const uint64_t *step, uint32_t key0,
                                                     uint32_t key1, bf16 *X, bf16 *Xt, float *Y,
                                                     const int64_t *fixed_rows) {
-    __shared__ int64_t rows[64];
-    __shared__ bf16 tile[64][66];  // [column][row], padded against bank conflicts
-    const int tx = static_cast<int>(threadIdx.x & 63u), ty = static_cast<int>(threadIdx.x >> 6);
-    const int m0 = static_cast<int>(blockIdx.x) * 64;
+    __shared__ int64_t rows[kGatherRows];
+    __shared__ bf16 tile[128][kGatherRows + 2];  // [column][row] of one 128-column slab
+    const int tid = static_cast<int>(threadIdx.x);
+    const int m0 = static_cast<int>(blockIdx.x) * kGatherRows;
     const int h = static_cast<int>(blockIdx.y);
-    if (ty == 0) {
-        const int m = m0 + tx;
+    if (tid < kGatherRows) {
+        const int m = m0 + tid;
         int64_t idx;
         if (fixed_rows) {
             idx = fixed_rows[m];
@@ -329,26 +330,30 @@ __global__ __launch_bounds__(256) void gather_batch(const float *xd, const float
                                            static_cast<uint32_t>(step[0]), 0x4D4C5031u, key0, key1);
             idx = static_cast<int64_t>((static_cast<uint64_t>(v.v0) * static_cast<uint64_t>(n)) >> 32);
         }
-        rows[tx] = idx;
+        rows[tid] = idx;
     }
     __syncthreads();
     bf16 *Xh = X + static_cast<long long>(h) * B * in_pad;
     bf16 *Xth = Xt + static_cast<long long>(h) * B * in_pad;
     float *Yh = Y + static_cast<long long>(h) * B * out_pad;
-    for (int j0 = 0; j0 < in_pad; j0 += 64) {
-        const int j = j0 + tx;
-        for (int i = ty; i < 64; i += 4) {
+    for (int j0 = 0; j0 < in_pad; j0 += 128) {
+        // 16 rows x 128 columns: thread -> column tid % 128, rows tid / 128 + 2 k
+        const int j = j0 + (tid & 127);
+        for (int i = tid >> 7; i < kGatherRows; i += 2) {
             const bf16 v = to_bf16(j < in ? xd[rows[i] * in + j] : 0.0f);
             Xh[static_cast<long long>(m0 + i) * in_pad + j] = v;
-            tile[tx][i] = v;
+            tile[tid & 127][i] = v;
         }
         __syncthreads();
-        for (int i = ty; i < 64; i += 4) Xth[static_cast<long long>(j0 + i) * B + m0 + tx] = tile[i][tx];
+        // Xt rows j0 .. j0 + 127, 16 rows of X each: thread -> (column tid / 16 + 16 k, row tid % 16)
+        for (int c = tid >> 4; c < 128; c += 16)
+            Xth[static_cast<long long>(j0 + c) * B + m0 + (tid & 15)] = tile[c][tid & 15];
         __syncthreads();
     }
-    for (int i = ty; i < 64; i += 4)
-        for (int j = tx; j < out_pad; j += 64)
-            Yh[static_cast<long long>(m0 + i) * out_pad + j] = j < out ? yd[rows[i] * out + j] : 0.0f;
+    for (int e = tid; e < kGatherRows * out_pad; e += 256) {
+        const int i = e / out_pad, j = e % out_pad;
+        Yh[static_cast<long long>(m0 + i) * out_pad + j] = j < out ? yd[rows[i] * out + j] : 0.0f;
+    }
 }
 
 // Weighted squared error over the real outputs on a 64-row x 64-column tile of head h: writes dY
@@ -957,7 +962,8 @@ class MlpModel {
     void enqueue_step(const int64_t *fixed_rows, bool update) {
         const uint32_t k0 = static_cast<uint32_t>(cfg_.seed ^ 0x5DEECE66DULL);
         const uint32_t k1 = static_cast<uint32_t>((cfg_.seed >> 32) ^ 0xB5297A4DULL);
-        hipLaunchKernelGGL(mlp::gather_batch, dim3(B_ / 64, H_), dim3(256), 0, stream_, xd_, yd_, n_data_,
+        hipLaunchKernelGGL(mlp::gather_batch, dim3(B_ / mlp::kGatherRows, H_), dim3(256), 0, stream_, xd_,
+                           yd_, n_data_,
                            cfg_.inputs, cfg_.outputs, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
                            Yt_, fixed_rows);
         MLP_TRY(hipGetLastError());
