@@ -46,6 +46,8 @@ enum {
 #define TMC_FLAG_NO_PROFILES 2u   /* diagnostic: skip the A_z/R_r/T_r/A_rz profiles (zeros) */
 #define TMC_FLAG_NO_GRAPH 4u      /* batched runs: launch directly instead of replaying a graph */
 #define TMC_FLAG_ANGLES 8u        /* also tally the exit-angle profiles into tmc_out.ra/ta/rra/tra */
+#define TMC_FLAG_AGGREGATE 16u    /* combine a wave's equal-bin A_z and A(r,z) adds (ballot + DPP
+                                   * sum) before the LDS atomics; results are identical */
 
 typedef struct tmc_layer {
     double n;   /* refractive index */
@@ -126,7 +128,7 @@ typedef struct tmc_batched_opts {
     int32_t device;
     double dz, dr;             /* one grid for every stack */
     int32_t nz, nr;
-    uint32_t flags;            /* TMC_FLAG_NO_PROFILES, TMC_FLAG_NO_GRAPH */
+    uint32_t flags;            /* TMC_FLAG_NO_PROFILES, TMC_FLAG_NO_GRAPH, TMC_FLAG_AGGREGATE */
     int32_t waves_per_group, groups, chunk; /* 0 = automatic */
     uint64_t step_limit;       /* 0 = 1e9 */
     uint64_t photons_hint;     /* expected photons per execute (sizes the claim chunk); 0 = auto */

@@ -27,6 +27,7 @@ void usage(std::FILE *to) {
                  "  --device D           HIP device ordinal (default 0)\n"
                  "  --arz                also tally A(r,z)\n"
                  "  --angles             also tally the exit-angle profiles (na bins of the .mci)\n"
+                 "  --aggregate          combine a wave's equal-bin profile adds before the LDS atomics\n"
                  "  --mco                write MCML's .mco file for each run, named as in the .mci\n"
                  "                       (implies --arz --angles)\n"
                  "  --no-profiles        diagnostic: skip the depth and radius profiles\n"
@@ -107,6 +108,8 @@ int run_command(int argc, char **argv) {
             opt.flags |= TMC_FLAG_ARZ;
         } else if (a == "--angles") {
             opt.flags |= TMC_FLAG_ANGLES;
+        } else if (a == "--aggregate") {
+            opt.flags |= TMC_FLAG_AGGREGATE;
         } else if (a == "--mco") {
             write_mco_files = true;
             opt.flags |= TMC_FLAG_ARZ | TMC_FLAG_ANGLES;
@@ -133,9 +136,10 @@ int run_command(int argc, char **argv) {
             return 2;
         }
     }
-    if ((opt.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES)) && (opt.flags & TMC_FLAG_NO_PROFILES)) {
-        std::fprintf(stderr, "tissuemc: --arz, --angles and --mco cannot be combined with "
-                             "--no-profiles\n");
+    if ((opt.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES | TMC_FLAG_AGGREGATE)) &&
+        (opt.flags & TMC_FLAG_NO_PROFILES)) {
+        std::fprintf(stderr, "tissuemc: --arz, --angles, --mco and --aggregate cannot be combined "
+                             "with --no-profiles\n");
         return 2;
     }
 

@@ -169,6 +169,88 @@ __device__ __forceinline__ void hbm_add(uint64_t *p, uint64_t v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------------------------ lane aggregation
+//
+// TMC_FLAG_AGGREGATE: the wave-level counterpart of the reference's aggregated_add
+// (gpu_tally.hpp:200-234, match_any + shuffles over a 32-lane warp). Wave64 has no match
+// instruction, so groups are peeled with ballots: the lowest lane still to score names its bin,
+// a ballot finds the lanes with that bin, a DPP reduction sums their weights, and that one lane
+// makes a single atomic add for the group. kAggPeels groups are peeled per score; the lanes left
+// after that add one by one. Integer sums do not depend on how they are grouped, so the tallies
+// are bit-identical to the plain path's. Every lane of the wave must be active.
+constexpr int kAggPeels = 1;
+
+// Sum of v over the wave's 64 lanes (every lane active), returned wave-uniform. DPP steps:
+// row_shr 1, 2, 3 of v (zero past the start of a 16-lane row), then row_shr 4 into banks 1-3 and
+// row_shr 8 into banks 2-3 of the running sum (lane 15 of each row now holds the row's sum),
+// row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3: lane 63 holds the total.
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    const int x = static_cast<int>(v);
+    int t = x;
+    t += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+    t += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+    t += __builtin_amdgcn_update_dpp(0, x, 0x113, 0xf, 0xf, true);
+    t += __builtin_amdgcn_update_dpp(0, t, 0x114, 0xf, 0xe, false);
+    t += __builtin_amdgcn_update_dpp(0, t, 0x118, 0xf, 0xc, false);
+    t += __builtin_amdgcn_update_dpp(0, t, 0x142, 0xa, 0xf, false);
+    t += __builtin_amdgcn_update_dpp(0, t, 0x143, 0xc, 0xf, false);
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(t, 63));
+}
+
+// Sum of fixed-point weights q <= 2^32 over the wave, exact: summed as 16-bit low parts and
+// high parts (each sum < 2^23), then recombined.
+__device__ __forceinline__ uint64_t wave_sum_fixed(uint64_t q) {
+    const uint32_t lo = wave_sum_u32(static_cast<uint32_t>(q) & 0xffffu);
+    const uint32_t hi = wave_sum_u32(static_cast<uint32_t>(q >> 16));
+    return (static_cast<uint64_t>(hi) << 16) + lo;
+}
+
+// Adds q into row[key] for the lanes with `score`, peeling kAggPeels groups first.
+__device__ __forceinline__ void aggregate_lds_add(uint64_t *row, bool score, int key, uint64_t q,
+                                                  int lane) {
+    bool pending = score;
+    uint64_t rest = __ballot(pending);
+    for (int p = 0; p < kAggPeels && rest != 0; ++p) {
+        const int leader = static_cast<int>(__builtin_ctzll(rest));
+        const int kb = __builtin_amdgcn_readlane(key, leader);
+        const bool in = pending && key == kb;
+        const uint64_t sum = wave_sum_fixed(in ? q : 0ull);
+        if (lane == leader) lds_add(row + kb, sum);
+        pending = pending && !in;
+        rest &= ~__ballot(in);
+    }
+    if (pending) lds_add(row + key, q);
+}
+
+// A(r,z) tile add of v < 2^38 into the 32-bit counter t, whose HBM bin is `bin`: the low word
+// goes to the counter, the high word and the counter's wrap (the atomic returns the value it
+// added to) to the HBM bin.
+__device__ __forceinline__ void tile_add(uint32_t *tile, uint64_t *arz, uint32_t t, uint32_t bin,
+                                         uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(v);
+    const uint32_t old =
+        __hip_atomic_fetch_add(tile + t, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint64_t carry = (v >> 32) + (old + lo < old ? 1u : 0u);
+    if (carry != 0) hbm_add(arz + bin, carry << 32);
+}
+
+__device__ __forceinline__ void aggregate_tile_add(uint32_t *tile, uint64_t *arz, bool score,
+                                                   uint32_t t, uint32_t bin, uint64_t q, int lane) {
+    bool pending = score;
+    uint64_t rest = __ballot(pending);
+    for (int p = 0; p < kAggPeels && rest != 0; ++p) {
+        const int leader = static_cast<int>(__builtin_ctzll(rest));
+        const uint32_t tb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t), leader));
+        const uint32_t gb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bin), leader));
+        const bool in = pending && t == tb;
+        const uint64_t sum = wave_sum_fixed(in ? q : 0ull);
+        if (lane == leader) tile_add(tile, arz, tb, gb, sum);
+        pending = pending && !in;
+        rest &= ~__ballot(in);
+    }
+    if (pending) tile_add(tile, arz, t, bin, q);
+}
+
 // The score checks of the reference (gpu_tally.hpp:87-94): a NaN weight or a direction whose
 // squared norm is off by more than 1e-5 counts as an anomaly.
 __device__ __forceinline__ uint32_t bad_score(float ux, float uy, float uz, float w) {
@@ -274,7 +356,7 @@ __device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, 
     }
 }
 
-template <bool kProfiles, bool kArz>
+template <bool kProfiles, bool kArz, bool kAgg>
 __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     extern __shared__ uint64_t lds[];
     const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -414,6 +496,12 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             z = __builtin_fmaf(s, uz, z);
 
             bool died = false;
+            // With kAgg the event's A_z and A(r,z)-tile scores are made after the branch, where
+            // every lane of the wave is active.
+            bool agg_z = false, agg_t = false;
+            int agg_iz = 0;
+            uint32_t agg_ti = 0, agg_bin = 0;
+            uint64_t agg_q = 0;
             if (hit) {
                 // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
                 ++bs;
@@ -505,7 +593,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 uint32_t tile_old = 0, tile_bin = 0;
                 if (kProfiles && alive) {
                     const int iz = bin_of(z, wu.inv_dz, last_z);
-                    lds_add(slot + off_az + iz, q);
+                    if (kAgg) {
+                        agg_z = true;
+                        agg_iz = iz;
+                        agg_q = q;
+                    } else {
+                        lds_add(slot + off_az + iz, q);
+                    }
                     if (kArz) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                         const int ir = bin_of(rr, wu.inv_dr, last_r);
@@ -515,10 +609,16 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                             const uint32_t t = __umul24(static_cast<uint32_t>(iz),
                                                         static_cast<uint32_t>(a.tile_r)) +
                                                static_cast<uint32_t>(ir);
-                            tile_old = __hip_atomic_fetch_add(tile + t, static_cast<uint32_t>(q),
-                                                              __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                            tile_pending = true;
+                            if (kAgg) {
+                                agg_t = true;
+                                agg_ti = t;
+                                agg_bin = tile_bin;
+                            } else {
+                                tile_old = __hip_atomic_fetch_add(tile + t, static_cast<uint32_t>(q),
+                                                                  __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                                tile_pending = true;
+                            }
                         } else {
                             hbm_add(a.arz + tile_bin, q);
                         }
@@ -567,6 +667,10 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                     const uint64_t carry = (q >> 32) + (tile_old + lo < tile_old ? 1u : 0u);
                     if (carry != 0) hbm_add(a.arz + tile_bin, carry << 32);
                 }
+            }
+            if (kAgg) {
+                aggregate_lds_add(slot + off_az, agg_z, agg_iz, agg_q, lane);
+                if (kArz) aggregate_tile_add(tile, a.arz, agg_t, agg_ti, agg_bin, agg_q, lane);
             }
             // roulette (physics.hpp:250-259, 274-276)
             if (!died && w < 1e-4f) {
@@ -725,8 +829,8 @@ DeviceInfo device_info(int device) {
     return d;
 }
 
-template <bool P, bool R> const void *kernel_ptr() {
-    return reinterpret_cast<const void *>(&transport_kernel<P, R>);
+template <bool P, bool R, bool A> const void *kernel_ptr() {
+    return reinterpret_cast<const void *>(&transport_kernel<P, R, A>);
 }
 
 }  // namespace
@@ -741,6 +845,7 @@ struct Problem {
     int lmax = 1, abs_stride = 1, nz = 1, nr = 1;
     size_t n_srows = 0, n_prows = 0;
     bool profiles = true, arz = false;
+    bool aggregate = false;  // lane aggregation of the A_z / A(r,z) adds
     int na = 0;  // exit-angle bins tallied (0: none)
     uint64_t photons = 0;
     int reach_z = 1;  // depth bins that can receive weight (for the A(r,z) tile)
@@ -762,6 +867,7 @@ class Resident {
                                      : 1000000000u;
         profiles_ = p.profiles;
         arz_ = p.arz;
+        agg_ = p.aggregate;
         a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
         a_.wave_words = ((a_.slot_words + 1) & ~1) + p.lmax * kLayerFields / 2;
 
@@ -956,8 +1062,9 @@ class Resident {
         return p;
     }
     const void *kernel() const {
-        if (!profiles_) return kernel_ptr<false, false>();
-        return arz_ ? kernel_ptr<true, true>() : kernel_ptr<true, false>();
+        if (!profiles_) return kernel_ptr<false, false, false>();
+        if (agg_) return arz_ ? kernel_ptr<true, true, true>() : kernel_ptr<true, false, true>();
+        return arz_ ? kernel_ptr<true, true, false>() : kernel_ptr<true, false, false>();
     }
     void enqueue_clear() {
         const unsigned grid = static_cast<unsigned>(std::min<size_t>((out_words_ + 255) / 256, 4096));
@@ -968,17 +1075,21 @@ class Resident {
     void enqueue_transport() {
         const dim3 grid(static_cast<unsigned>(groups_)), block(static_cast<unsigned>(wpg_ * 64));
         if (!profiles_)
-            hipLaunchKernelGGL((transport_kernel<false, false>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<false, false, false>), grid, block, lds_bytes_, stream_, a_);
+        else if (agg_ && arz_)
+            hipLaunchKernelGGL((transport_kernel<true, true, true>), grid, block, lds_bytes_, stream_, a_);
+        else if (agg_)
+            hipLaunchKernelGGL((transport_kernel<true, false, true>), grid, block, lds_bytes_, stream_, a_);
         else if (arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, true, false>), grid, block, lds_bytes_, stream_, a_);
         else
-            hipLaunchKernelGGL((transport_kernel<true, false>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, false, false>), grid, block, lds_bytes_, stream_, a_);
         HIP_TRY(hipGetLastError());
     }
 
     DeviceInfo dev_;
     KernelArgs a_;
-    bool profiles_ = true, arz_ = false;
+    bool profiles_ = true, arz_ = false, agg_ = false;
     int wpg_ = 16, groups_ = 0;
     size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
     void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
@@ -1005,6 +1116,9 @@ tmc_spec checked_single(const tmc_spec &spec_in, const tmc_opts &opt) {
     const bool profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
     if ((opt.flags & TMC_FLAG_ARZ) != 0 && !profiles)
         throw EngineError(TMC_ERR_CONFIG, "A(r,z) is a profile: it cannot be combined with no-profiles");
+    if ((opt.flags & TMC_FLAG_AGGREGATE) != 0 && !profiles)
+        throw EngineError(TMC_ERR_CONFIG, "lane aggregation applies to the profiles: it cannot be "
+                                          "combined with no-profiles");
     if ((opt.flags & TMC_FLAG_ANGLES) != 0) {
         if (!profiles)
             throw EngineError(TMC_ERR_CONFIG,
@@ -1025,6 +1139,7 @@ Problem single_problem(const tmc_spec &spec, const tmc_opts &opt) {
     p.nr = spec.nr;
     p.profiles = (opt.flags & TMC_FLAG_NO_PROFILES) == 0;
     p.arz = (opt.flags & TMC_FLAG_ARZ) != 0;
+    p.aggregate = (opt.flags & TMC_FLAG_AGGREGATE) != 0;
     p.na = (opt.flags & TMC_FLAG_ANGLES) != 0 ? spec.na : 0;
     p.photons = spec.photons;
     p.n_srows = B;
@@ -1179,6 +1294,10 @@ BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &o) {
     p.nr = o.nr;
     p.profiles = (o.flags & TMC_FLAG_NO_PROFILES) == 0;
     p.arz = false;
+    if ((o.flags & TMC_FLAG_AGGREGATE) != 0 && !p.profiles)
+        throw EngineError(TMC_ERR_CONFIG, "lane aggregation applies to the profiles: it cannot be "
+                                          "combined with no-profiles");
+    p.aggregate = (o.flags & TMC_FLAG_AGGREGATE) != 0;
     p.n_srows = static_cast<size_t>(n_stacks) * o.batches_per_stack;
     p.n_prows = static_cast<size_t>(n_stacks);
     p.stack_stride = kStackHeader + kMaxLayers * kLayerFields;

@@ -18,7 +18,7 @@ MAX_LAYERS = 16
 UNIT = 2.0 ** -32  # weight of one tally unit
 
 TMC_OK, TMC_ERR_CONFIG, TMC_ERR_DEVICE, TMC_ERR_RUNTIME = 0, 1, 2, 3
-FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH, FLAG_ANGLES = 1, 2, 4, 8
+FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH, FLAG_ANGLES, FLAG_AGGREGATE = 1, 2, 4, 8, 16
 
 
 class ConfigError(ValueError):
@@ -243,12 +243,12 @@ def batch_estimate(sums, photons) -> tuple[float, float]:
 
 
 def _opts(*, seed, batches, device, photons, stream, arz, profiles, waves_per_group=0, groups=0,
-          chunk=0, arz_tile=None, step_limit=0, angles=False) -> TmcOpts:
+          chunk=0, arz_tile=None, step_limit=0, angles=False, aggregate=False) -> TmcOpts:
     o = TmcOpts()
     load().tmc_default_opts(C.byref(o))
     o.seed, o.batches, o.device, o.photons, o.stream = seed, int(batches), device, photons, stream
     o.flags = ((FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
-               | (FLAG_ANGLES if angles else 0))
+               | (FLAG_ANGLES if angles else 0) | (FLAG_AGGREGATE if aggregate else 0))
     o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
     if arz_tile is not None:
         o.arz_tile_z, o.arz_tile_r = arz_tile
@@ -264,7 +264,7 @@ def _report(r: TmcTuneReport) -> dict:
 
 def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
              arz: bool = False, profiles: bool = True, stream: int = 0, chunk: int = 0,
-             step_limit: int = 0, pilot_fraction: float = 1 / 32) -> dict:
+             step_limit: int = 0, pilot_fraction: float = 1 / 32, aggregate: bool = False) -> dict:
     """Pick the launch shape of a single run by timing pilot runs (tmc_autotune).
 
     Returns ``{"run_kwargs": ..., "report": ...}``; ``run(spec, **run_kwargs, ...)`` uses the
@@ -273,7 +273,8 @@ def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, 
     L = load()
     cs = spec_to_c(spec)
     o = _opts(seed=seed, batches=batches, device=device, photons=photons or int(spec["photons"]),
-              stream=stream, arz=arz, profiles=profiles, chunk=chunk, step_limit=step_limit)
+              stream=stream, arz=arz, profiles=profiles, chunk=chunk, step_limit=step_limit,
+              aggregate=aggregate)
     tuned, rep = TmcOpts(), TmcTuneReport()
     code = L.tmc_autotune(C.byref(cs), C.byref(o), float(pilot_fraction), C.byref(tuned),
                           C.byref(rep))
@@ -288,7 +289,7 @@ def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, 
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
         arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
         groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
-        step_limit: int = 0, angles: bool = False) -> dict:
+        step_limit: int = 0, angles: bool = False, aggregate: bool = False) -> dict:
     """Simulate one run on the GPU; returns the reference's result schema plus extras.
 
     Extras: ``t_r``, ``a_rz`` (if ``arz``), the exit-angle profiles ``rd_a``/``t_a`` [na] and
@@ -301,7 +302,7 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     B, NL, nz, nr = int(batches), cs.num_layers, cs.nz, cs.nr
     o = _opts(seed=seed, batches=B, device=device, photons=n_ph, stream=stream, arz=arz,
               profiles=profiles, waves_per_group=waves_per_group, groups=groups, chunk=chunk,
-              arz_tile=arz_tile, step_limit=step_limit, angles=angles)
+              arz_tile=arz_tile, step_limit=step_limit, angles=angles, aggregate=aggregate)
     if B < 1:
         raise ConfigError("batches must be >= 1")
     na = max(int(cs.na), 1)
@@ -450,11 +451,13 @@ class BatchedPlan:
 
     def __init__(self, n_stacks: int, *, batches_per_stack: int = 10, dz: float = 0.01,
                  dr: float = 0.01, nz: int = 100, nr: int = 100, device: int = 0,
-                 profiles: bool = True, graph: bool = True, waves_per_group: int = 0,
+                 profiles: bool = True, graph: bool = True, aggregate: bool = False,
+                 waves_per_group: int = 0,
                  groups: int = 0, chunk: int = 0, step_limit: int = 0, photons_hint: int = 0):
         self.n_stacks, self.bs, self.nz, self.nr = n_stacks, batches_per_stack, nz, nr
         self._h = C.c_void_p()
-        flags = (0 if profiles else FLAG_NO_PROFILES) | (0 if graph else FLAG_NO_GRAPH)
+        flags = ((0 if profiles else FLAG_NO_PROFILES) | (0 if graph else FLAG_NO_GRAPH)
+                 | (FLAG_AGGREGATE if aggregate else 0))
         o = TmcBatchedOpts(seed=1, batches_per_stack=batches_per_stack, device=device, dz=dz,
                            dr=dr, nz=nz, nr=nr, flags=flags,
                            waves_per_group=waves_per_group, groups=groups, chunk=chunk,
