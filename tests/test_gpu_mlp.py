@@ -34,9 +34,12 @@ def data(n=512, seed=3):
 def test_param_count_and_flops(model):
     assert model.param_count == M.param_count(*DIMS)
     per_step, per_query = model.flops()
-    # padded shapes: in 30 -> 128, out 202 -> 256
-    q = 2 * (128 * 128 + 128 * 128 * 2 + 256 * 128) * 2
+    # algorithmic: the real widths (30 inputs, 202 outputs), not the padded ones
+    q = 2 * (30 * 128 + 128 * 128 * 2 + 202 * 128) * 2
     assert per_query == q
+    # a step: forward and weight gradient of every layer, input gradient of all but the first
+    per_head = 2 * CFG["batch"] * (30 * 128 * 2 + 128 * 128 * 2 * 3 + 202 * 128 * 3)
+    assert per_step == per_head * 2
 
 
 def test_forward_matches_oracle(model):

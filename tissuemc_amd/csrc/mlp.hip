@@ -76,7 +76,9 @@ __device__ __forceinline__ float gelu_grad(float z) {
     return __builtin_fmaf(z * s * (1.0f - s), du, s);
 }
 
-enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kBwdWeight = 3 };
+// kInferHidden is kFwdHidden for inference: no Z and no transposed copy, so its results need no
+// LDS staging (below).
+enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kBwdWeight = 3, kInferHidden = 4 };
 
 struct GemmArgs {
     const bf16 *A;
@@ -110,12 +112,23 @@ struct GemmArgs {
 // c ^ s(r), s(r) = ((r >> 1) ^ (r >> 2)) & 3: the 16 rows of a fragment land in distinct banks
 // whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle. The loads pick their global
 // chunk to match, since a lane's destination in LDS is fixed by its lane number.
+//
+// Epilogues that write bf16 H / dZ together with their transposes stage the tile through LDS
+// (68 KB, so two workgroups per CU). The others store straight from registers and use only the
+// ring; for those the MFMA operands are swapped (the tile is computed as B A^T), so that a lane
+// holds four consecutive columns of one row: 16 bytes of fp32, or 8 bytes of bf16, per store.
+// Their ring has three stages (48 KB): three workgroups per CU, which the registers also allow.
 constexpr int kBK = 32;                          // K per stage
-constexpr int kStages = 4;                       // LDS ring depth
 constexpr int kStageElems = 2 * 128 * kBK;       // bf16 elements of one stage (A then B)
 constexpr int kPitch = 136;                      // bf16 row pitch of the epilogue's output tiles
-constexpr int kGemmLds = 2 * 128 * kPitch * 2;   // bytes: the larger of staging and epilogue use
-static_assert(kGemmLds >= kStages * kStageElems * 2, "epilogue tiles must cover the staging ring");
+constexpr int kGemmLds = 2 * 128 * kPitch * 2;   // bytes: the epilogue tiles of H and H^T
+constexpr bool lds_epilogue(int epi) { return epi == kFwdHidden || epi == kBwdData; }
+constexpr bool swapped(int epi) { return epi == kInferHidden || epi == kFwdOut; }
+constexpr int stages_of(int epi) { return lds_epilogue(epi) ? 4 : 3; }
+constexpr int lds_bytes_of(int epi) {
+    return lds_epilogue(epi) ? kGemmLds : stages_of(epi) * kStageElems * 2;
+}
+static_assert(kGemmLds >= 4 * kStageElems * 2, "epilogue tiles must cover the staging ring");
 
 __device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
@@ -134,6 +147,7 @@ __device__ __forceinline__ void stage_sync() {
 
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
+    constexpr int kStages = stages_of(kEpi);
     extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // staging ring, then T and Tt
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
@@ -170,11 +184,18 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     const int ch = (kq ^ swz(r)) * 8;
     const int arow = ((wave >> 1) * 64 + r) * kBK + ch, brow = ((wave & 1) * 64 + r) * kBK + ch;
 
+    // Swapped tiles start from the bias (a lane's four columns are consecutive), which takes the
+    // bias add out of the epilogue.
     floatx4 acc[4][4];
+    const int g4 = (lane >> 4) * 4;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+        floatx4 init = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (swapped(kEpi))
+            init = *reinterpret_cast<const floatx4 *>(g.bias + h * g.sBias + n0 + g4 + 16 * j);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 4; ++i) acc[i][j] = init;
+    }
 
 #define TMC_GEMM_ISSUE(st)                                                                        \
     do {                                                                                          \
@@ -191,7 +212,7 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     for (int st = 0; st < kStages - 1 && st < nk; ++st) TMC_GEMM_ISSUE(st);
     for (int kt = 0; kt < nk; ++kt) {
         // stages issued after kt so far: min(kStages - 2, nk - 1 - kt), four loads each
-        const int ahead = nk - 1 - kt;
+        const int ahead = nk - 1 - kt < kStages - 2 ? nk - 1 - kt : kStages - 2;
         if (ahead >= 2) stage_sync<8>();
         else if (ahead == 1) stage_sync<4>();
         else stage_sync<0>();
@@ -207,9 +228,35 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = swapped(kEpi)
+                                ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0)
+                                : __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
 #undef TMC_GEMM_ISSUE
+    if constexpr (swapped(kEpi)) {
+        // In each 16 x 16 tile the lane holds row m = m0 + 16 i + (lane % 16) and the four
+        // columns nb .. nb + 3, nb = n0 + 16 j + 4 (lane / 16), bias included. Every value the
+        // ring held has been read, and nothing here touches LDS.
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long row = static_cast<long long>(m0 + 16 * i + r) * g.N + n0 + g4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if constexpr (kEpi == kInferHidden) {
+                    bf16 o[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) o[q] = to_bf16(gelu(acc[i][j][q]));
+                    uint2 packed;
+                    packed.x = static_cast<uint32_t>(o[0]) | (static_cast<uint32_t>(o[1]) << 16);
+                    packed.y = static_cast<uint32_t>(o[2]) | (static_cast<uint32_t>(o[3]) << 16);
+                    *reinterpret_cast<uint2 *>(g.Cb + h * g.sCb + row + 16 * j) = packed;
+                } else {  // kFwdOut
+                    *reinterpret_cast<floatx4 *>(g.Cf + h * g.sCf + row + 16 * j) = acc[i][j];
+                }
+            }
+        }
+        return;
+    }
     stage_sync<0>();  // every wave is done with the ring before the epilogue reuses the LDS
 
     // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
@@ -277,16 +324,9 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
             for (int j = 0; j < 4; ++j) {
                 const int n = n0 + 16 * j + r;
                 const int mb = m0 + 16 * i + (lane >> 4) * 4;
-                if (kEpi == kBwdWeight) {
-                    float *C = g.Cf + h * g.sCf + split * g.sSplit;
+                float *C = g.Cf + h * g.sCf + split * g.sSplit;  // kBwdWeight
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
-                } else {  // kFwdOut
-                    float *C = g.Cf + h * g.sCf;
-                    const float b = g.bias[h * g.sBias + n];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q] + b;
-                }
+                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
             }
         }
     }
@@ -502,13 +542,23 @@ namespace tmc {
 namespace mlp {
 
 // Inputs for prediction: the same rows for every head, bf16, zero-padded.
+// Queries -> bf16 network inputs X [B x in_pad] (rows past `rows` and columns past `in` are
+// zero), shared by every head: one thread per 8 columns, one 16-byte store.
 __global__ void load_inputs(const float *xf, int rows, int in, int in_pad, int B, bf16 *X) {
-    const int m = static_cast<int>(blockIdx.x);
-    const int h = static_cast<int>(blockIdx.y);
-    for (int j = static_cast<int>(threadIdx.x); j < in_pad; j += static_cast<int>(blockDim.x)) {
-        const float v = (m < rows && j < in) ? xf[static_cast<long long>(m) * in + j] : 0.0f;
-        X[(static_cast<long long>(h) * B + m) * in_pad + j] = to_bf16(v);
+    const int per_row = in_pad / 8;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(B) * per_row) return;
+    const int m = static_cast<int>(t / per_row), j0 = static_cast<int>(t % per_row) * 8;
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = j0 + 2 * k;
+        const bool ok = m < rows;
+        const uint32_t lo = ok && j < in ? to_bf16(xf[static_cast<long long>(m) * in + j]) : 0u;
+        const uint32_t hi = ok && j + 1 < in ? to_bf16(xf[static_cast<long long>(m) * in + j + 1]) : 0u;
+        w[k] = lo | (hi << 16);
     }
+    *reinterpret_cast<uint4 *>(X + static_cast<long long>(m) * in_pad + j0) = uint4{w[0], w[1], w[2], w[3]};
 }
 
 // Ensemble mean of every output and the spread (standard deviation over heads) of the first
@@ -562,16 +612,14 @@ class MlpModel {
         MLP_TRY(hipSetDevice(c.device));
         // the GEMM's LDS (staging tiles, then the epilogue's output tiles) exceeds the 64 KB that a
         // launch gets without asking
-        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kFwdHidden>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
-        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kFwdOut>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
-        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kBwdData>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
-        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<mlp::kBwdWeight>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kGemmLds));
+        set_lds<mlp::kFwdHidden>();
+        set_lds<mlp::kFwdOut>();
+        set_lds<mlp::kBwdData>();
+        set_lds<mlp::kBwdWeight>();
+        set_lds<mlp::kInferHidden>();
         H_ = c.heads;
         in_pad_ = pad128(c.inputs);
+        k0_ = (c.inputs + mlp::kBK - 1) / mlp::kBK * mlp::kBK;
         out_pad_ = pad128(c.outputs);
         hid_ = c.hidden;
         B_ = c.batch;
@@ -748,8 +796,8 @@ class MlpModel {
             MLP_TRY(hipMemcpyAsync(xq_, x + r0 * cfg_.inputs, static_cast<size_t>(rows) * cfg_.inputs * 4,
                                    hipMemcpyHostToDevice, stream_));
             MLP_TRY(hipEventRecord(e0_, stream_));
-            hipLaunchKernelGGL(mlp::load_inputs, dim3(Bq_, H_), dim3(64), 0, stream_, xq_, rows,
-                               cfg_.inputs, in_pad_, Bq_, Xq_);
+            hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (in_pad_ / 8) + 255) / 256), dim3(256), 0,
+                               stream_, xq_, rows, cfg_.inputs, in_pad_, Bq_, Xq_);
             forward(Xq_, Bq_, Hq_, nullptr, nullptr, true);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3(rows), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
@@ -789,15 +837,16 @@ class MlpModel {
 
     void set_lr(float lr) { MLP_TRY(hipMemcpy(lr_, &lr, 4, hipMemcpyHostToDevice)); }
     int heads() const { return H_; }
+    // Algorithmic FLOPs, on the network's real widths (the kernels also multiply padding).
     uint64_t flops_per_step() const {
         uint64_t f = 0;
         for (size_t k = 0; k < mats_.size(); ++k)
-            f += 2ull * B_ * mats_[k].out * mats_[k].in * (k == 0 ? 2 : 3);  // fwd + dW (+ dX)
+            f += 2ull * B_ * static_cast<uint64_t>(real_out_[k]) * real_in_[k] * (k == 0 ? 2 : 3);  // fwd + dW (+ dX)
         return f * H_;
     }
     uint64_t flops_per_query() const {
         uint64_t f = 0;
-        for (const auto &mi : mats_) f += 2ull * mi.out * mi.in;
+        for (size_t k = 0; k < mats_.size(); ++k) f += 2ull * real_out_[k] * real_in_[k];
         return f * H_;
     }
 
@@ -871,12 +920,17 @@ class MlpModel {
         MLP_TRY(hipStreamSynchronize(stream_));
     }
 
+    template <int kEpi> static void set_lds() {
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_nt<kEpi>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    mlp::lds_bytes_of(kEpi)));
+    }
     template <int kEpi> void gemm(mlp::GemmArgs g) {
         if (g.splits < 1) g.splits = 1;
         if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
             throw EngineError(TMC_ERR_RUNTIME, "gemm_nt shape is not a multiple of the tile");
         hipLaunchKernelGGL((mlp::gemm_nt<kEpi>), dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256),
-                           mlp::kGemmLds, stream_, g);
+                           mlp::lds_bytes_of(kEpi), stream_, g);
         MLP_TRY(hipGetLastError());
     }
 
@@ -922,13 +976,15 @@ class MlpModel {
             mlp::GemmArgs g = base_args();
             g.A = in;
             g.lda = in_dim;
-            g.sA = static_cast<long long>(Bn) * in_dim;
+            // inference feeds every head the same queries
+            g.sA = infer && l == 0 ? 0 : static_cast<long long>(Bn) * in_dim;
             g.B = wb_ + mi.wt_off;
             g.ldb = mi.in;
             g.sB = Pb_;
             g.M = Bn;
             g.N = hid_;
-            g.K = in_dim;
+            // the first layer's inputs are zero past k0_ (padding): its products stop there
+            g.K = l == 0 ? k0_ : in_dim;
             g.bias = p_ + mi.boff;
             g.sBias = P_;
             g.Cf = infer ? nullptr : Zbuf[l];
@@ -936,7 +992,10 @@ class MlpModel {
             g.Cb = infer ? Hbuf[l % 2] : Hbuf[l];
             g.Ct = infer ? nullptr : Htbuf[l];
             g.sCb = BH;
-            gemm<mlp::kFwdHidden>(g);
+            if (infer)
+                gemm<mlp::kInferHidden>(g);
+            else
+                gemm<mlp::kFwdHidden>(g);
             in = g.Cb;
             in_dim = hid_;
         }
@@ -1057,6 +1116,7 @@ class MlpModel {
 
     tmc_mlp_config cfg_;
     int H_ = 1, in_pad_ = 128, out_pad_ = 128, hid_ = 128, B_ = 128, Bq_ = 128;
+    int k0_ = 128;  // K of the first layer: the inputs rounded up to a stage (32)
     long long P_ = 0, Pb_ = 0;
     std::vector<mlp::MatInfo> mats_;
     std::vector<int> real_in_, real_out_;
