@@ -48,6 +48,12 @@ __host__ __device__ __forceinline__ bf16 to_bf16(float f) {
     return static_cast<bf16>(u >> 16);
 #endif
 }
+// Two floats -> two bf16 in one word (lo in the low half), nearest even: one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float float2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2_t{lo, hi}, bf16x2_t));
+}
 __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
     const uint32_t u = static_cast<uint32_t>(h) << 16;
     float f;
@@ -243,13 +249,9 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if constexpr (kEpi == kInferHidden) {
-                    bf16 o[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) o[q] = to_bf16(gelu(acc[i][j][q]));
-                    uint2 packed;
-                    packed.x = static_cast<uint32_t>(o[0]) | (static_cast<uint32_t>(o[1]) << 16);
-                    packed.y = static_cast<uint32_t>(o[2]) | (static_cast<uint32_t>(o[3]) << 16);
-                    *reinterpret_cast<uint2 *>(g.Cb + h * g.sCb + row + 16 * j) = packed;
+                    const floatx4 &z = acc[i][j];
+                    *reinterpret_cast<uint2 *>(g.Cb + h * g.sCb + row + 16 * j) =
+                        uint2{pack_bf16x2(gelu(z[0]), gelu(z[1])), pack_bf16x2(gelu(z[2]), gelu(z[3]))};
                 } else {  // kFwdOut
                     *reinterpret_cast<floatx4 *>(g.Cf + h * g.sCf + row + 16 * j) = acc[i][j];
                 }
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
             for (int j = 0; j < 4; ++j) {
                 const int col = wn + 16 * j + r, rb = wm + 16 * i + (lane >> 4) * 4;
                 const int n = nblk + col, mb = mblk + rb;
-                bf16 o[4];
+                float v[4];
                 if (kEpi == kFwdHidden) {
                     float *Zp = g.Cf ? g.Cf + h * g.sCf : nullptr;
                     const float b = g.bias[h * g.sBias + n];
@@ -283,22 +285,20 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
                     for (int q = 0; q < 4; ++q) {
                         const float zv = acc[i][j][q] + b;
                         if (Zp) Zp[static_cast<long long>(mb + q) * g.N + n] = zv;
-                        o[q] = to_bf16(gelu(zv));
+                        v[q] = gelu(zv);
                     }
                 } else {  // kBwdData
                     const float *Zp = g.Zin + h * g.sZin;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        o[q] = to_bf16(acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]));
+                        v[q] = acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]);
                 }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) T[(rb + q) * kPitch + col] = o[q];
-                if (Ct) {
-                    uint2 packed;
-                    packed.x = static_cast<uint32_t>(o[0]) | (static_cast<uint32_t>(o[1]) << 16);
-                    packed.y = static_cast<uint32_t>(o[2]) | (static_cast<uint32_t>(o[3]) << 16);
-                    *reinterpret_cast<uint2 *>(Tt + col * kPitch + rb) = packed;
-                }
+                const uint32_t p01 = pack_bf16x2(v[0], v[1]), p23 = pack_bf16x2(v[2], v[3]);
+                T[rb * kPitch + col] = static_cast<bf16>(p01);
+                T[(rb + 1) * kPitch + col] = static_cast<bf16>(p01 >> 16);
+                T[(rb + 2) * kPitch + col] = static_cast<bf16>(p23);
+                T[(rb + 3) * kPitch + col] = static_cast<bf16>(p23 >> 16);
+                if (Ct) *reinterpret_cast<uint2 *>(Tt + col * kPitch + rb) = uint2{p01, p23};
             }
         }
         __syncthreads();
