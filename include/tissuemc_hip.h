@@ -48,6 +48,8 @@ enum {
 #define TMC_FLAG_ANGLES 8u        /* also tally the exit-angle profiles into tmc_out.ra/ta/rra/tra */
 #define TMC_FLAG_AGGREGATE 16u    /* combine a wave's equal-bin A_z and A(r,z) adds (ballot + DPP
                                    * sum) before the LDS atomics; results are identical */
+#define TMC_FLAG_TIME 32u         /* also tally R_d(t), T_t(t) into tmc_out.rt/tt_t (tmc_opts.nt,
+                                   * dt); not with TMC_FLAG_AGGREGATE */
 
 typedef struct tmc_layer {
     double n;   /* refractive index */
@@ -81,6 +83,11 @@ typedef struct tmc_opts {
     int32_t arz_tile_z;      /* depth bins of A(r,z) kept in LDS; -1 = automatic */
     int32_t arz_tile_r;      /* radial bins of A(r,z) kept in LDS; -1 = automatic */
     uint64_t step_limit;     /* steps after which a photon is ended as an anomaly; 0 = 1e9 */
+    /* With TMC_FLAG_TIME (no reference counterpart): time-of-flight bins of the escaping weight,
+     * nt bins of dt ns from launch (the last bin takes later photons); time = optical path / c,
+     * the optical path summing n of the layer times each hop's length. */
+    double dt;               /* bin width [ns] */
+    int32_t nt;              /* bin count */
 } tmc_opts;
 
 /* Results. Array pointers are caller-owned host buffers sized by B = batches, L = num_layers,
@@ -105,6 +112,8 @@ typedef struct tmc_out {
      * escaping at an angle from the normal in na bins of pi / (2 na), all batches. */
     uint64_t *ra, *ta;        /* [na]    diffuse reflectance, transmittance per angle bin */
     uint64_t *rra, *tra;      /* [nr*na] per (radius, angle) bin, row-major in radius */
+    /* With TMC_FLAG_TIME: escaping weight per time-of-flight bin, all batches. */
+    uint64_t *rt, *tt_t;      /* [nt]    diffuse reflectance, transmittance per time bin */
 } tmc_out;
 
 /* ---------------------------------------------------------------- batched multi-stack runs

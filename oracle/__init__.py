@@ -40,7 +40,8 @@ class OrcSpec(C.Structure):
 
 class OrcOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("batches", C.c_int32), ("threads", C.c_int32),
-                ("precision", C.c_int32), ("stream", C.c_uint32), ("photons", C.c_uint64)]
+                ("precision", C.c_int32), ("stream", C.c_uint32), ("photons", C.c_uint64),
+                ("nt", C.c_int32), ("dt", C.c_double)]
 
 
 _P64 = C.POINTER(C.c_uint64)
@@ -51,7 +52,7 @@ class OrcOut(C.Structure):
     _fields_ = [("first", _P64), ("count", _P64), ("rd", _PD), ("tt", _PD), ("absorbed", _PD),
                 ("events", _P64), ("anomalies", _P64), ("boundary_steps", _P64), ("az", _PD),
                 ("rr", _PD), ("tr", _PD), ("arz", _PD), ("ra", _PD), ("ta", _PD), ("rra", _PD),
-                ("tra", _PD), ("specular", C.c_double),
+                ("tra", _PD), ("rt", _PD), ("tt_t", _PD), ("specular", C.c_double),
                 ("threads_used", C.c_int32), ("wall_seconds", C.c_double)]
 
 
@@ -148,16 +149,21 @@ def _ptr(a: np.ndarray, t):
 
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threads: int = 0,
         precision: str = "float64", stream: int = 0, arz: bool = False,
-        angles: bool = False) -> dict:
+        angles: bool = False, time: tuple[int, float] | None = None) -> dict:
     """Simulate one run with the oracle; returns a result dict in the reference's schema.
 
     With ``angles`` the exit-angle profiles are added: ``rd_a``/``t_a`` [na] and ``rd_ra``/
-    ``t_ra`` [nr, na] (sums over batches), per batch under ``ra_b``/``ta_b``."""
+    ``t_ra`` [nr, na] (sums over batches), per batch under ``ra_b``/``ta_b``. With ``time =
+    (nt, dt)`` the time-of-flight profiles of the escaping weight: ``rd_t``/``t_t`` [nt] in bins
+    of dt ns (time = optical path / c), per batch under ``rt_b``/``tt_b``."""
     s = to_spec(spec)
     n_ph = photons or s.photons
     B, L, nz, nr = batches, s.num_layers, s.nz, s.nr
+    nt, dt = (int(time[0]), float(time[1])) if time is not None else (0, 0.0)
     o = OrcOpts(seed=seed, batches=batches, threads=threads,
-                precision=32 if precision == "float32" else 64, stream=stream, photons=n_ph)
+                precision=32 if precision == "float32" else 64, stream=stream, photons=n_ph,
+                nt=nt, dt=dt)
+    tof = [np.zeros(B * max(nt, 1)), np.zeros(B * max(nt, 1))] if time is not None else [None] * 2
     first = np.zeros(B, np.uint64)
     count = np.zeros(B, np.uint64)
     rd = np.zeros(B)
@@ -182,7 +188,9 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threa
                  ra=_ptr(ang[0], C.c_double) if angles else None,
                  ta=_ptr(ang[1], C.c_double) if angles else None,
                  rra=_ptr(ang[2], C.c_double) if angles else None,
-                 tra=_ptr(ang[3], C.c_double) if angles else None)
+                 tra=_ptr(ang[3], C.c_double) if angles else None,
+                 rt=_ptr(tof[0], C.c_double) if time is not None else None,
+                 tt_t=_ptr(tof[1], C.c_double) if time is not None else None)
     err = C.create_string_buffer(256)
     if lib().orc_run(C.byref(s), C.byref(o), C.byref(out), err, 256) != 0:
         raise ValueError(err.value.decode())
@@ -221,6 +229,9 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, threa
         res["rd_a"], res["t_a"] = res["ra_b"].sum(axis=0), res["ta_b"].sum(axis=0)
         res["rd_ra"] = ang[2].reshape(B, nr, na).sum(axis=0)
         res["t_ra"] = ang[3].reshape(B, nr, na).sum(axis=0)
+    if time is not None:
+        res["rt_b"], res["tt_b"] = tof[0].reshape(B, nt), tof[1].reshape(B, nt)
+        res["rd_t"], res["t_t"] = res["rt_b"].sum(axis=0), res["tt_b"].sum(axis=0)
     return res
 
 

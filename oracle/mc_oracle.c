@@ -64,6 +64,8 @@ typedef struct {
     int32_t precision; /* 64 or 32 */
     uint32_t stream;   /* counter word 3; 0 = the reference stream */
     uint64_t photons;  /* nonzero overrides spec->photons */
+    int32_t nt;        /* time-of-flight bins (with out->rt); no reference counterpart */
+    double dt;         /* their width [ns] */
 } orc_opts;
 
 /* Caller-owned output arrays; B = batches, L = layers. Any profile pointer may be NULL. */
@@ -78,6 +80,7 @@ typedef struct {
     double *arz;                       /* [B*nz*nr] */
     double *ra, *ta;                   /* [B*na] exit-angle profiles, or NULL */
     double *rra, *tra;                 /* [B*nr*na] (radius, angle), or NULL with ra */
+    double *rt, *tt_t;                 /* [B*nt] time-of-flight profiles, or NULL */
     double specular;
     int32_t threads_used;
     double wall_seconds;
@@ -193,6 +196,7 @@ typedef struct {
     csum absorbed[ORC_MAX_LAYERS];
     csum *az, *rr, *tr, *arz;
     csum *ra, *ta, *rra, *tra;
+    csum *rt, *tt_t;
     uint64_t events, anomalies, boundary_steps;
 } orc_tally;
 
@@ -316,6 +320,11 @@ static void run_batch(orc_job *job, uint64_t b) {
         t.rra = calloc((size_t)nr * (size_t)na, sizeof(csum));
         t.tra = calloc((size_t)nr * (size_t)na, sizeof(csum));
     }
+    const int nt = job->opt->nt;
+    if (out->rt) {
+        t.rt = calloc((size_t)nt, sizeof(csum));
+        t.tt_t = calloc((size_t)nt, sizeof(csum));
+    }
     const uint64_t first = plan_first(&job->plan, b), count = plan_count(&job->plan, b);
     for (uint64_t id = first; id < first + count; ++id) {
         orc_rng rng;
@@ -360,8 +369,16 @@ static void run_batch(orc_job *job, uint64_t b) {
     free(t.arz);
     free(t.ra);
     free(t.ta);
+    if (out->rt) {
+        for (int i = 0; i < nt; ++i) {
+            out->rt[b * (uint64_t)nt + (uint64_t)i] = csum_value(&t.rt[i]);
+            out->tt_t[b * (uint64_t)nt + (uint64_t)i] = csum_value(&t.tt_t[i]);
+        }
+    }
     free(t.rra);
     free(t.tra);
+    free(t.rt);
+    free(t.tt_t);
 }
 
 static void *worker(void *arg) {
@@ -394,6 +411,9 @@ int orc_run(const orc_spec *spec_in, const orc_opts *opt, orc_out *out, char *er
     if (out->ra && (!out->ta || !out->rra || !out->tra))
         return fail_msg(err, errlen, "angle profiles need ra, ta, rra and tra");
     if (out->ra && run.na < 1) return fail_msg(err, errlen, "angle profiles need na >= 1");
+    if (out->rt && !out->tt_t) return fail_msg(err, errlen, "time profiles need rt and tt_t");
+    if (out->rt && (opt->nt < 1 || !(opt->dt > 0.0)))
+        return fail_msg(err, errlen, "time profiles need nt >= 1 and dt > 0");
     int threads = opt->threads;
     if (threads == 0) {
         const long hw = sysconf(_SC_NPROCESSORS_ONLN);
@@ -409,6 +429,9 @@ int orc_run(const orc_spec *spec_in, const orc_opts *opt, orc_out *out, char *er
     job.plan = plan_make(run.photons, (uint64_t)opt->batches);
     make_geometry_d(&run, &job.geo_d);
     make_geometry_f(&run, &job.geo_f);
+    /* a time bin of dt ns is c dt cm of optical path (c = 29.9792458 cm/ns) */
+    job.geo_d.nt = job.geo_f.nt = out->rt ? opt->nt : 0;
+    job.geo_d.dt_opl = job.geo_f.dt_opl = out->rt ? 29.9792458 * opt->dt : 1.0;
     atomic_init(&job.next_batch, 0);
     out->specular = specular_d(&job.geo_d);
 

@@ -28,6 +28,7 @@ void usage(std::FILE *to) {
                  "  --arz                also tally A(r,z)\n"
                  "  --angles             also tally the exit-angle profiles (na bins of the .mci)\n"
                  "  --aggregate          combine a wave's equal-bin profile adds before the LDS atomics\n"
+                 "  --time NT DT         also tally R_d(t), T_t(t): NT bins of DT ns of flight time\n"
                  "  --mco                write MCML's .mco file for each run, named as in the .mci\n"
                  "                       (implies --arz --angles)\n"
                  "  --no-profiles        diagnostic: skip the depth and radius profiles\n"
@@ -110,6 +111,16 @@ int run_command(int argc, char **argv) {
             opt.flags |= TMC_FLAG_ANGLES;
         } else if (a == "--aggregate") {
             opt.flags |= TMC_FLAG_AGGREGATE;
+        } else if (a == "--time") {
+            opt.flags |= TMC_FLAG_TIME;
+            opt.nt = static_cast<int32_t>(parse_count("--time", value("--time")));
+            const std::string dt = value("--time");
+            char *end = nullptr;
+            opt.dt = std::strtod(dt.c_str(), &end);
+            if (end == dt.c_str() || *end != '\0' || !(opt.dt > 0.0)) {
+                std::fprintf(stderr, "tissuemc: --time needs a bin count and a bin width > 0 (ns)\n");
+                return 2;
+            }
         } else if (a == "--mco") {
             write_mco_files = true;
             opt.flags |= TMC_FLAG_ARZ | TMC_FLAG_ANGLES;
@@ -136,10 +147,10 @@ int run_command(int argc, char **argv) {
             return 2;
         }
     }
-    if ((opt.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES | TMC_FLAG_AGGREGATE)) &&
+    if ((opt.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES | TMC_FLAG_AGGREGATE | TMC_FLAG_TIME)) &&
         (opt.flags & TMC_FLAG_NO_PROFILES)) {
-        std::fprintf(stderr, "tissuemc: --arz, --angles, --mco and --aggregate cannot be combined "
-                             "with --no-profiles\n");
+        std::fprintf(stderr, "tissuemc: --arz, --angles, --mco, --aggregate and --time cannot be "
+                             "combined with --no-profiles\n");
         return 2;
     }
 

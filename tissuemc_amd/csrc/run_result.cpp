@@ -112,9 +112,12 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
     const bool arz = (opts.flags & TMC_FLAG_ARZ) != 0;
     const bool angles = (opts.flags & TMC_FLAG_ANGLES) != 0;
     const size_t na = static_cast<size_t>(spec.na > 0 ? spec.na : 0);
+    const bool time = (opts.flags & TMC_FLAG_TIME) != 0;
+    const size_t nt = static_cast<size_t>(opts.nt > 0 ? opts.nt : 0);
     std::vector<uint64_t> rd(B), tt(B), ab(B * L), ev(B), an(B), bs(B), az(B * nz), rr(B * nr),
         tr(B * nr), a_rz(arz ? nz * nr : 0), ra(angles ? na : 0), ta(angles ? na : 0),
-        rra(angles ? nr * na : 0), tra(angles ? nr * na : 0);
+        rra(angles ? nr * na : 0), tra(angles ? nr * na : 0), rt(time ? nt : 0),
+        ttt(time ? nt : 0);
     tmc_out out{};
     out.rd = rd.data();
     out.tt = tt.data();
@@ -131,6 +134,10 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
         out.ta = ta.data();
         out.rra = rra.data();
         out.tra = tra.data();
+    }
+    if (time) {
+        out.rt = rt.data();
+        out.tt_t = ttt.data();
     }
     const int code = tmc_hip_run(&spec, &opts, &out);
     if (code != TMC_OK) rethrow(code);
@@ -197,6 +204,11 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
         r.t_a = scaled(ta);
         r.rd_ra = scaled(rra);
         r.t_ra = scaled(tra);
+    }
+    if (time) {
+        r.rd_t = scaled(rt);
+        r.t_t = scaled(ttt);
+        r.dt_ns = opts.dt;
     }
     return r;
 }
@@ -450,6 +462,17 @@ void write_json(const RunResult &run, std::ostream &out) {
         j.raw(",");
         j.key("a_rz");
         j.array(run.a_rz);
+    }
+    if (!run.rd_t.empty()) {
+        j.raw(",");
+        j.key("time_dt_ns");
+        j.num(run.dt_ns);
+        j.raw(",");
+        j.key("rd_t");
+        j.array(run.rd_t);
+        j.raw(",");
+        j.key("t_t");
+        j.array(run.t_t);
     }
     if (!run.rd_a.empty()) {
         const std::pair<const char *, const std::vector<double> *> angle[] = {

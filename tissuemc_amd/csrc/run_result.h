@@ -2,8 +2,8 @@
 //
 // RunResult keeps the reference's fields and meanings (include/tissuemc/result.hpp:15-59) so
 // the JSON a run writes is a superset of the reference's schema (src/io/json_out.cpp:164-306).
-// Additions: T_r, A(r,z), the exit-angle profiles, per-bin standard errors of the profiles,
-// boundary-step counts and the launch shape. The GPU's integer tallies (units of 2^-32) are
+// Additions: T_r, A(r,z), the exit-angle and time-of-flight profiles, per-bin standard errors of
+// the profiles, boundary-step counts and the launch shape. The GPU's integer tallies (units of 2^-32) are
 // summed as integers, then scaled.
 #pragma once
 
@@ -50,6 +50,8 @@ struct RunResult {
     std::vector<double> a_rz;                     // [nz*nr] or empty
     std::vector<double> rd_a, t_a;                // [na] exit-angle profiles, or empty
     std::vector<double> rd_ra, t_ra;              // [nr*na], row-major in radius, or empty
+    std::vector<double> rd_t, t_t;                // [nt] time-of-flight profiles, or empty
+    double dt_ns = 0;                             // their bin width
     double wall_seconds = 0, kernel_seconds = 0;
     std::string device;
     std::string tally_precision = "int64 fixed point (2^-32)";

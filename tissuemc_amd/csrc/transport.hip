@@ -106,6 +106,8 @@ enum StackHeader { kHdrW0 = 0, kHdrInvDz, kHdrInvDr, kHdrLayers, kStackHeader };
 // absorbed[lmax], then A_z[nz].
 enum SlotScalar { kSlotRd = 0, kSlotTt, kSlotEvents, kSlotAnomalies, kSlotBoundary, kSlotAbs };
 
+constexpr double kLightCmPerNs = 29.9792458;  // speed of light in vacuum
+
 // A unit of work: a range of photon ids of one stack and the rows its results add into.
 struct UnitRec {
     uint64_t first;  // first photon id (ids index the stack's stream)
@@ -125,6 +127,9 @@ struct KernelArgs {
     uint64_t *cursor;     // [1] units handed out so far
     uint64_t *rd, *tt, *absorbed, *events, *anomalies, *bsteps, *az, *rr, *tr, *arz;
     uint64_t *ra, *ta, *rra, *tra;  // exit-angle profiles [na], [nr x na]; na == 0: none
+    uint64_t *rt, *ttt;             // time-resolved R_d(t), T_t(t) [nt]; nt == 0: none
+    int32_t nt;
+    float inv_dopl;                 // 1 / (c dt): optical path (cm) -> time bin
     uint32_t n_units, stack_stride;
     int32_t lmax, abs_stride, nz, nr, na;
     float inv_da;  // 2 na / pi
@@ -356,7 +361,7 @@ __device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, 
     }
 }
 
-template <bool kProfiles, bool kArz, bool kAgg>
+template <bool kProfiles, bool kArz, bool kAgg, bool kTime>
 __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     extern __shared__ uint64_t lds[];
     const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -376,6 +381,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     const int off_az = kSlotAbs + a.lmax;
     const float last_z = static_cast<float>(a.nz - 1), last_r = static_cast<float>(a.nr - 1);
     const float last_a = static_cast<float>(a.na - 1);
+    const float last_t = static_cast<float>(a.nt - 1);
     const uint64_t seed = a.key[0];
     const uint32_t key0 = static_cast<uint32_t>(seed), key1 = static_cast<uint32_t>(seed >> 32);
 
@@ -398,6 +404,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
     uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
     uint64_t acc = 0;
+    float opl = 0;  // kTime: optical path length so far (cm): sum of n s over the hops
 
     while (have) {
         for (;;) {
@@ -433,6 +440,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 uz = start ? 1.0f : uz;
                 w = start ? wu.w0 : w;
                 sleft = start ? 0.0f : sleft;
+                if (kTime) opl = start ? 0.0f : opl;
                 layer = start ? 0 : layer;
                 nstep = start ? 0u : nstep;
                 acc = start ? 0ull : acc;
@@ -494,6 +502,9 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             x = __builtin_fmaf(s, ux, x);
             y = __builtin_fmaf(s, uy, y);
             z = __builtin_fmaf(s, uz, z);
+            // time of flight: the hop's optical path, n s, added as a product then a sum (the
+            // oracle's float32 arithmetic, so both bin the same photons alike)
+            if (kTime) opl = opl + s * P.f[kN];
 
             bool died = false;
             // With kAgg the event's A_z and A(r,z)-tile scores are made after the branch, where
@@ -567,6 +578,11 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                                          static_cast<uint32_t>(a.na)) +
                                                 ia,
                                             qw);
+                                }
+                                if (kTime) {
+                                    // time of flight opl / c, in nt bins of dt, clamped
+                                    const int it = bin_of(opl, a.inv_dopl, last_t);
+                                    hbm_add((exit_top ? a.rt : a.ttt) + it, qw);
                                 }
                             }
                         }
@@ -829,8 +845,8 @@ DeviceInfo device_info(int device) {
     return d;
 }
 
-template <bool P, bool R, bool A> const void *kernel_ptr() {
-    return reinterpret_cast<const void *>(&transport_kernel<P, R, A>);
+template <bool P, bool R, bool A, bool T> const void *kernel_ptr() {
+    return reinterpret_cast<const void *>(&transport_kernel<P, R, A, T>);
 }
 
 }  // namespace
@@ -847,6 +863,8 @@ struct Problem {
     bool profiles = true, arz = false;
     bool aggregate = false;  // lane aggregation of the A_z / A(r,z) adds
     int na = 0;  // exit-angle bins tallied (0: none)
+    int nt = 0;  // time bins tallied (0: none)
+    double dt = 0.0;  // their width (ns)
     uint64_t photons = 0;
     int reach_z = 1;  // depth bins that can receive weight (for the A(r,z) tile)
 };
@@ -868,6 +886,7 @@ class Resident {
         profiles_ = p.profiles;
         arz_ = p.arz;
         agg_ = p.aggregate;
+        time_ = p.nt > 0;
         a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
         a_.wave_words = ((a_.slot_words + 1) & ~1) + p.lmax * kLayerFields / 2;
 
@@ -922,7 +941,8 @@ class Resident {
         out_words_ = 1 /*cursor*/ + U /*next*/ + 5 * S + S * p.abs_stride +
                      (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
                      (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
-                     (p.na > 0 ? 2 * static_cast<size_t>(p.na) * (1 + p.nr) : 0);
+                     (p.na > 0 ? 2 * static_cast<size_t>(p.na) * (1 + p.nr) : 0) +
+                     2 * static_cast<size_t>(p.nt);
         out_ = alloc(out_words_ * 8);
         uint64_t *q = static_cast<uint64_t *>(out_);
         a_.cursor = q; q += 1;
@@ -948,7 +968,14 @@ class Resident {
             a_.ra = q; q += p.na;
             a_.ta = q; q += p.na;
             a_.rra = q; q += static_cast<size_t>(p.nr) * p.na;
-            a_.tra = q;
+            a_.tra = q; q += static_cast<size_t>(p.nr) * p.na;
+        }
+        a_.nt = p.nt;
+        // a time bin of dt ns is c dt cm of optical path (c = 29.9792458 cm/ns)
+        a_.inv_dopl = p.nt > 0 ? static_cast<float>(1.0 / (kLightCmPerNs * p.dt)) : 0.0f;
+        if (p.nt > 0) {
+            a_.rt = q; q += p.nt;
+            a_.ttt = q; q += p.nt;
         }
         a_.units = static_cast<const UnitRec *>(units_);
         a_.stacks = static_cast<const float *>(stacks_);
@@ -1016,7 +1043,8 @@ class Resident {
     void download(uint64_t *rd, uint64_t *tt, uint64_t *absorbed, uint64_t *events,
                   uint64_t *anomalies, uint64_t *bsteps, uint64_t *az, uint64_t *rr, uint64_t *tr,
                   uint64_t *arz, uint64_t *ra = nullptr, uint64_t *ta = nullptr,
-                  uint64_t *rra = nullptr, uint64_t *tra = nullptr) {
+                  uint64_t *rra = nullptr, uint64_t *tra = nullptr, uint64_t *rt = nullptr,
+                  uint64_t *ttt = nullptr) {
         auto d2h = [&](uint64_t *dst, const uint64_t *src, size_t n) {
             if (dst && src && n) HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream_));
         };
@@ -1043,6 +1071,10 @@ class Resident {
             d2h(rra, a_.rra, static_cast<size_t>(a_.nr) * a_.na);
             d2h(tra, a_.tra, static_cast<size_t>(a_.nr) * a_.na);
         }
+        if (a_.nt > 0) {
+            d2h(rt, a_.rt, a_.nt);
+            d2h(ttt, a_.ttt, a_.nt);
+        }
         HIP_TRY(hipStreamSynchronize(stream_));
     }
 
@@ -1062,9 +1094,10 @@ class Resident {
         return p;
     }
     const void *kernel() const {
-        if (!profiles_) return kernel_ptr<false, false, false>();
-        if (agg_) return arz_ ? kernel_ptr<true, true, true>() : kernel_ptr<true, false, true>();
-        return arz_ ? kernel_ptr<true, true, false>() : kernel_ptr<true, false, false>();
+        if (!profiles_) return kernel_ptr<false, false, false, false>();
+        if (time_) return arz_ ? kernel_ptr<true, true, false, true>() : kernel_ptr<true, false, false, true>();
+        if (agg_) return arz_ ? kernel_ptr<true, true, true, false>() : kernel_ptr<true, false, true, false>();
+        return arz_ ? kernel_ptr<true, true, false, false>() : kernel_ptr<true, false, false, false>();
     }
     void enqueue_clear() {
         const unsigned grid = static_cast<unsigned>(std::min<size_t>((out_words_ + 255) / 256, 4096));
@@ -1075,21 +1108,25 @@ class Resident {
     void enqueue_transport() {
         const dim3 grid(static_cast<unsigned>(groups_)), block(static_cast<unsigned>(wpg_ * 64));
         if (!profiles_)
-            hipLaunchKernelGGL((transport_kernel<false, false, false>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<false, false, false, false>), grid, block, lds_bytes_, stream_, a_);
+        else if (time_ && arz_)
+            hipLaunchKernelGGL((transport_kernel<true, true, false, true>), grid, block, lds_bytes_, stream_, a_);
+        else if (time_)
+            hipLaunchKernelGGL((transport_kernel<true, false, false, true>), grid, block, lds_bytes_, stream_, a_);
         else if (agg_ && arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true, true>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, true, true, false>), grid, block, lds_bytes_, stream_, a_);
         else if (agg_)
-            hipLaunchKernelGGL((transport_kernel<true, false, true>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, false, true, false>), grid, block, lds_bytes_, stream_, a_);
         else if (arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true, false>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, true, false, false>), grid, block, lds_bytes_, stream_, a_);
         else
-            hipLaunchKernelGGL((transport_kernel<true, false, false>), grid, block, lds_bytes_, stream_, a_);
+            hipLaunchKernelGGL((transport_kernel<true, false, false, false>), grid, block, lds_bytes_, stream_, a_);
         HIP_TRY(hipGetLastError());
     }
 
     DeviceInfo dev_;
     KernelArgs a_;
-    bool profiles_ = true, arz_ = false, agg_ = false;
+    bool profiles_ = true, arz_ = false, agg_ = false, time_ = false;
     int wpg_ = 16, groups_ = 0;
     size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
     void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
@@ -1119,6 +1156,17 @@ tmc_spec checked_single(const tmc_spec &spec_in, const tmc_opts &opt) {
     if ((opt.flags & TMC_FLAG_AGGREGATE) != 0 && !profiles)
         throw EngineError(TMC_ERR_CONFIG, "lane aggregation applies to the profiles: it cannot be "
                                           "combined with no-profiles");
+    if ((opt.flags & TMC_FLAG_TIME) != 0) {
+        if (!profiles)
+            throw EngineError(TMC_ERR_CONFIG,
+                              "the time-resolved profiles cannot be combined with no-profiles");
+        if ((opt.flags & TMC_FLAG_AGGREGATE) != 0)
+            throw EngineError(TMC_ERR_CONFIG,
+                              "the time-resolved profiles cannot be combined with lane aggregation");
+        if (opt.nt < 1) throw EngineError(TMC_ERR_CONFIG, "the time-resolved profiles need nt >= 1");
+        if (!(opt.dt > 0.0) || !std::isfinite(opt.dt))
+            throw EngineError(TMC_ERR_CONFIG, "the time bin width dt must be > 0 (ns)");
+    }
     if ((opt.flags & TMC_FLAG_ANGLES) != 0) {
         if (!profiles)
             throw EngineError(TMC_ERR_CONFIG,
@@ -1141,6 +1189,8 @@ Problem single_problem(const tmc_spec &spec, const tmc_opts &opt) {
     p.arz = (opt.flags & TMC_FLAG_ARZ) != 0;
     p.aggregate = (opt.flags & TMC_FLAG_AGGREGATE) != 0;
     p.na = (opt.flags & TMC_FLAG_ANGLES) != 0 ? spec.na : 0;
+    p.nt = (opt.flags & TMC_FLAG_TIME) != 0 ? opt.nt : 0;
+    p.dt = opt.dt;
     p.photons = spec.photons;
     p.n_srows = B;
     p.n_prows = B;
@@ -1187,6 +1237,9 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     const bool want_angles = (opt.flags & TMC_FLAG_ANGLES) != 0;
     if (want_angles && (!out->ra || !out->ta || !out->rra || !out->tra))
         throw EngineError(TMC_ERR_CONFIG, "angle profiles requested but tmc_out.ra/ta/rra/tra is NULL");
+    const bool want_time = (opt.flags & TMC_FLAG_TIME) != 0;
+    if (want_time && (!out->rt || !out->tt_t))
+        throw EngineError(TMC_ERR_CONFIG, "time-resolved profiles requested but tmc_out.rt/tt_t is NULL");
     const DeviceInfo dev = device_info(opt.device);
 
     const Problem p = single_problem(spec, opt);
@@ -1196,7 +1249,8 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     r.download(out->rd, out->tt, out->absorbed, out->events, out->anomalies, out->boundary_steps,
                out->az, out->rr, out->tr, want_arz ? out->arz : nullptr,
                want_angles ? out->ra : nullptr, want_angles ? out->ta : nullptr,
-               want_angles ? out->rra : nullptr, want_angles ? out->tra : nullptr);
+               want_angles ? out->rra : nullptr, want_angles ? out->tra : nullptr,
+               want_time ? out->rt : nullptr, want_time ? out->tt_t : nullptr);
     {
         const double n0 = spec.n_above, n1 = spec.layers[0].n;
         const double rr = (n0 - n1) / (n0 + n1);
@@ -1281,9 +1335,10 @@ BatchedPlan *batched_create(int n_stacks, const tmc_batched_opts &o) {
         throw EngineError(TMC_ERR_CONFIG, "dz, dr must be > 0 and nz, nr >= 1");
     if (static_cast<uint64_t>(n_stacks) * o.batches_per_stack > 0xffffffffull)
         throw EngineError(TMC_ERR_CONFIG, "too many stacks x batches");
-    if ((o.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES)) != 0)
-        throw EngineError(TMC_ERR_CONFIG, "batched runs tally A_z, R_r and T_r per stack; A(r,z) "
-                                          "and the angle profiles are single-run options");
+    if ((o.flags & (TMC_FLAG_ARZ | TMC_FLAG_ANGLES | TMC_FLAG_TIME)) != 0)
+        throw EngineError(TMC_ERR_CONFIG, "batched runs tally A_z, R_r and T_r per stack; A(r,z), "
+                                          "the angle and the time-resolved profiles are single-run "
+                                          "options");
     auto plan = std::make_unique<BatchedPlan>();
     plan->dev = device_info(o.device);
     plan->opt = o;

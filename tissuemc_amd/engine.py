@@ -18,7 +18,7 @@ MAX_LAYERS = 16
 UNIT = 2.0 ** -32  # weight of one tally unit
 
 TMC_OK, TMC_ERR_CONFIG, TMC_ERR_DEVICE, TMC_ERR_RUNTIME = 0, 1, 2, 3
-FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH, FLAG_ANGLES, FLAG_AGGREGATE = 1, 2, 4, 8, 16
+FLAG_ARZ, FLAG_NO_PROFILES, FLAG_NO_GRAPH, FLAG_ANGLES, FLAG_AGGREGATE, FLAG_TIME = 1, 2, 4, 8, 16, 32
 
 
 class ConfigError(ValueError):
@@ -45,7 +45,8 @@ class TmcOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("batches", C.c_int32), ("device", C.c_int32),
                 ("photons", C.c_uint64), ("stream", C.c_uint32), ("flags", C.c_uint32),
                 ("waves_per_group", C.c_int32), ("groups", C.c_int32), ("chunk", C.c_int32),
-                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("step_limit", C.c_uint64)]
+                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("step_limit", C.c_uint64),
+                ("dt", C.c_double), ("nt", C.c_int32)]
 
 
 _PU64 = C.POINTER(C.c_uint64)
@@ -59,7 +60,7 @@ class TmcOut(C.Structure):
                 ("device", C.c_char * 96), ("groups", C.c_int32), ("waves_per_group", C.c_int32),
                 ("lds_bytes", C.c_int32), ("chunk", C.c_int32), ("arz_tile_z", C.c_int32),
                 ("arz_tile_r", C.c_int32), ("ra", _PU64), ("ta", _PU64), ("rra", _PU64),
-                ("tra", _PU64)]
+                ("tra", _PU64), ("rt", _PU64), ("tt_t", _PU64)]
 
 
 class TmcStack(C.Structure):
@@ -243,12 +244,16 @@ def batch_estimate(sums, photons) -> tuple[float, float]:
 
 
 def _opts(*, seed, batches, device, photons, stream, arz, profiles, waves_per_group=0, groups=0,
-          chunk=0, arz_tile=None, step_limit=0, angles=False, aggregate=False) -> TmcOpts:
+          chunk=0, arz_tile=None, step_limit=0, angles=False, aggregate=False,
+          time=None) -> TmcOpts:
     o = TmcOpts()
     load().tmc_default_opts(C.byref(o))
     o.seed, o.batches, o.device, o.photons, o.stream = seed, int(batches), device, photons, stream
     o.flags = ((FLAG_ARZ if arz else 0) | (0 if profiles else FLAG_NO_PROFILES)
-               | (FLAG_ANGLES if angles else 0) | (FLAG_AGGREGATE if aggregate else 0))
+               | (FLAG_ANGLES if angles else 0) | (FLAG_AGGREGATE if aggregate else 0)
+               | (FLAG_TIME if time is not None else 0))
+    if time is not None:
+        o.nt, o.dt = int(time[0]), float(time[1])
     o.waves_per_group, o.groups, o.chunk, o.step_limit = waves_per_group, groups, chunk, step_limit
     if arz_tile is not None:
         o.arz_tile_z, o.arz_tile_r = arz_tile
@@ -289,11 +294,13 @@ def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, 
 def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
         arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
         groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
-        step_limit: int = 0, angles: bool = False, aggregate: bool = False) -> dict:
+        step_limit: int = 0, angles: bool = False, aggregate: bool = False,
+        time: tuple[int, float] | None = None) -> dict:
     """Simulate one run on the GPU; returns the reference's result schema plus extras.
 
     Extras: ``t_r``, ``a_rz`` (if ``arz``), the exit-angle profiles ``rd_a``/``t_a`` [na] and
-    ``rd_ra``/``t_ra`` [nr, na] (if ``angles``), ``profile_se`` (per-bin standard errors),
+    ``rd_ra``/``t_ra`` [nr, na] (if ``angles``), the time-of-flight profiles ``rd_t``/``t_t``
+    [nt] (if ``time = (nt, dt_ns)``), ``profile_se`` (per-bin standard errors),
     per-batch profiles ``az_b``/``rr_b``/``tr_b`` and the raw integer tallies under ``fixed``.
     """
     L = load()
@@ -302,7 +309,8 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     B, NL, nz, nr = int(batches), cs.num_layers, cs.nz, cs.nr
     o = _opts(seed=seed, batches=B, device=device, photons=n_ph, stream=stream, arz=arz,
               profiles=profiles, waves_per_group=waves_per_group, groups=groups, chunk=chunk,
-              arz_tile=arz_tile, step_limit=step_limit, angles=angles, aggregate=aggregate)
+              arz_tile=arz_tile, step_limit=step_limit, angles=angles, aggregate=aggregate,
+              time=time)
     if B < 1:
         raise ConfigError("batches must be >= 1")
     na = max(int(cs.na), 1)
@@ -319,6 +327,9 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
                  arz=_p(a_rz) if arz else None)
     if angles:
         out.ra, out.ta, out.rra, out.tra = (_p(x) for x in ang)
+    tof = [np.zeros(max(int(time[0]), 1), np.uint64) for _ in range(2)] if time is not None else None
+    if time is not None:
+        out.rt, out.tt_t = _p(tof[0]), _p(tof[1])
     code = L.tmc_hip_run(C.byref(cs), C.byref(o), C.byref(out))
     if code != TMC_OK:
         _raise(code)
@@ -386,6 +397,11 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
         res["t_a"] = ang[1].astype(np.float64) * UNIT
         res["rd_ra"] = (ang[2].astype(np.float64) * UNIT).reshape(nr, na)
         res["t_ra"] = (ang[3].astype(np.float64) * UNIT).reshape(nr, na)
+    if time is not None:
+        res["fixed"].update(rt=tof[0], tt_t=tof[1])
+        res["rd_t"] = tof[0].astype(np.float64) * UNIT
+        res["t_t"] = tof[1].astype(np.float64) * UNIT
+        res["time_grid"] = {"nt": int(time[0]), "dt_ns": float(time[1])}
     return res
 
 
