@@ -336,12 +336,15 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 // elements [off, off + count) of each head's block (fixed order: deterministic).
 __global__ void reduce_splits(const float *part, int splits, long long count, long long s_part_head,
                               long long s_split, float *g, long long s_g_head) {
+    // four consecutive elements per thread (count, the strides and the offsets are multiples of
+    // 4 floats, so every access is a 16-byte load or store)
     const int h = static_cast<int>(blockIdx.y);
-    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < count;
-         e += static_cast<long long>(gridDim.x) * blockDim.x) {
-        float s = 0.0f;
-        for (int k = 0; k < splits; ++k) s += part[h * s_part_head + k * s_split + e];
-        g[h * s_g_head + e] = s;
+    for (long long e = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; e < count;
+         e += static_cast<long long>(gridDim.x) * blockDim.x * 4) {
+        floatx4 s = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < splits; ++k)
+            s += *reinterpret_cast<const floatx4 *>(part + h * s_part_head + k * s_split + e);
+        *reinterpret_cast<floatx4 *>(g + h * s_g_head + e) = s;
     }
 }
 
@@ -449,20 +452,24 @@ __global__ void loss_reduce(const float *part, int n, float inv_count, float *lo
 }
 
 // Bias gradient: db[h][n] = sum_m dZt[h][n][m] (rows of the transposed gradient are contiguous).
-__global__ void bias_grad(const bf16 *dZt, int B, int N, float *db, long long s_db) {
-    const int n = static_cast<int>(blockIdx.x);
+// One wave per row n, four rows per block: 16-byte loads of 8 bf16, then a butterfly of
+// shuffles in a fixed order (deterministic). B is a multiple of 8 and N of 4.
+__global__ __launch_bounds__(256) void bias_grad(const bf16 *dZt, int B, int N, float *db, long long s_db) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int n = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     const int h = static_cast<int>(blockIdx.y);
-    const bf16 *row = dZt + (static_cast<long long>(h) * N + n) * B;
+    const uint4 *row = reinterpret_cast<const uint4 *>(dZt + (static_cast<long long>(h) * N + n) * B);
     float s = 0.0f;
-    for (int m = static_cast<int>(threadIdx.x); m < B; m += static_cast<int>(blockDim.x)) s += from_bf16(row[m]);
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = static_cast<int>(blockDim.x) / 2; off > 0; off >>= 1) {
-        if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    for (int q = lane; q < B / 8; q += 64) {
+        const uint4 w = row[q];
+        const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            s += from_bf16(static_cast<bf16>(words[k])) + from_bf16(static_cast<bf16>(words[k] >> 16));
     }
-    if (threadIdx.x == 0) db[h * s_db + n] = red[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) db[h * s_db + n] = s;
 }
 
 struct MatInfo {
@@ -484,34 +491,64 @@ struct AdamArgs {
     const uint64_t *step;
 };
 
-// Adam (Kingma & Ba 2015) with bias correction; also refreshes the bf16 weight copies.
-__global__ void adam_update(const AdamArgs a) {
+// Adam (Kingma & Ba 2015) with bias correction for one element; returns the new parameter.
+__device__ __forceinline__ float adam_step(const AdamArgs &a, float *p, const float *g, float *m,
+                                           float *v, long long e, float c1, float c2) {
+    const float gr = g[e] + a.wd * p[e];
+    const float mn = a.beta1 * m[e] + (1.0f - a.beta1) * gr;
+    const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
+    m[e] = mn;
+    v[e] = vn;
+    const float pn = p[e] - a.lr[0] * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
+    p[e] = pn;
+    return pn;
+}
+
+// Adam on the weight matrices, one 64 x 64 tile of one matrix per block (blockIdx.x counts the
+// tiles of all matrices in order, blockIdx.y is the head), and the bf16 copies the products
+// read: W [out x in] row by row like the fp32 parameters, and its transpose [in x out] through
+// an LDS tile, so that every global access is coalesced.
+constexpr int kAdamTile = 64;
+__global__ __launch_bounds__(256) void adam_matrices(const AdamArgs a) {
+    __shared__ bf16 tile[kAdamTile][kAdamTile + 2];  // [i][o]
     const int h = static_cast<int>(blockIdx.y);
-    const float t = static_cast<float>(a.step[0] + 1);
-    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, t));
-    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, t));
-    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
-    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < a.P;
-         e += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const float gr = g[e] + a.wd * p[e];
-        const float mn = a.beta1 * m[e] + (1.0f - a.beta1) * gr;
-        const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
-        m[e] = mn;
-        v[e] = vn;
-        const float pn = p[e] - a.lr[0] * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
-        p[e] = pn;
-        for (int k = 0; k < a.n_mats; ++k) {
-            const MatInfo &mi = a.mats[k];
-            const long long rel = e - mi.off;
-            if (rel >= 0 && rel < static_cast<long long>(mi.out) * mi.in) {
-                const int o = static_cast<int>(rel / mi.in), i = static_cast<int>(rel % mi.in);
-                bf16 *wb = a.wb + h * a.Pb;
-                const bf16 b = to_bf16(pn);
-                wb[mi.wt_off + rel] = b;
-                wb[mi.w_off + static_cast<long long>(i) * mi.out + o] = b;
-            }
-        }
+    int t = static_cast<int>(blockIdx.x), k = 0;
+    for (; k < a.n_mats; ++k) {
+        const int n = (a.mats[k].out / kAdamTile) * (a.mats[k].in / kAdamTile);
+        if (t < n) break;
+        t -= n;
     }
+    const MatInfo mi = a.mats[k];
+    const int tiles_i = mi.in / kAdamTile;
+    const int o0 = (t / tiles_i) * kAdamTile, i0 = (t % tiles_i) * kAdamTile;
+    const float tt = static_cast<float>(a.step[0] + 1);
+    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, tt));
+    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, tt));
+    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
+    bf16 *wb = a.wb + h * a.Pb;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int c = tid & 63;
+    for (int r = tid >> 6; r < kAdamTile; r += 4) {  // row o0 + r, column i0 + c
+        const long long rel = static_cast<long long>(o0 + r) * mi.in + i0 + c;
+        const bf16 b = to_bf16(adam_step(a, p, g, m, v, mi.off + rel, c1, c2));
+        wb[mi.wt_off + rel] = b;
+        tile[c][r] = b;
+    }
+    __syncthreads();
+    for (int r = tid >> 6; r < kAdamTile; r += 4)  // transposed: row i0 + r, column o0 + c
+        wb[mi.w_off + static_cast<long long>(i0 + r) * mi.out + o0 + c] = tile[r][c];
+}
+
+// Adam on the biases: block x is a matrix, block y a head.
+__global__ __launch_bounds__(256) void adam_biases(const AdamArgs a) {
+    const int h = static_cast<int>(blockIdx.y);
+    const MatInfo mi = a.mats[blockIdx.x];
+    const float tt = static_cast<float>(a.step[0] + 1);
+    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, tt));
+    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, tt));
+    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
+    for (int o = static_cast<int>(threadIdx.x); o < mi.out; o += static_cast<int>(blockDim.x))
+        adam_step(a, p, g, m, v, mi.boff + o, c1, c2);
 }
 
 __global__ void bump_step(uint64_t *step) { step[0] += 1; }
@@ -651,7 +688,10 @@ class MlpModel {
         g_ = alloc<float>(H_ * P_);
         // split-K of the weight gradients: about 256 batch rows per split, each split a whole
         // number of 64-row GEMM stages
-        splits_ = std::max(1, B_ / 256);
+        // weight gradients split the batch (their K) into pieces of 1024: measured at batch 4096,
+        // 1024 beats 256, 512, 2048 and no split (0.447 ms a step against 0.498, 0.458, 0.492,
+        // 0.601): fewer partials to reduce, and still 256 workgroups for a hidden layer's product
+        splits_ = std::max(1, B_ / 1024);
         while (B_ % (mlp::kBK * splits_) != 0) --splits_;
         part_ = alloc<float>(static_cast<long long>(H_) * splits_ * P_);
         m_ = alloc<float>(H_ * P_);
@@ -953,7 +993,7 @@ class MlpModel {
         g.sSplit = P_;
         gemm<mlp::kBwdWeight>(g);
         const long long count = static_cast<long long>(mi.out) * mi.in;
-        hipLaunchKernelGGL(mlp::reduce_splits, dim3(static_cast<unsigned>(std::min<long long>((count + 255) / 256, 1024)), H_),
+        hipLaunchKernelGGL(mlp::reduce_splits, dim3(static_cast<unsigned>(std::min<long long>((count / 4 + 255) / 256, 1024)), H_),
                            dim3(256), 0, stream_, part_ + mi.off, splits_, count,
                            static_cast<long long>(splits_) * P_, P_, g_ + mi.off, P_);
         MLP_TRY(hipGetLastError());
@@ -1041,7 +1081,7 @@ class MlpModel {
         {
             const auto &mo = mats_[D];
             weight_grad(mo, dYt_, Ht_[D - 1], BH);
-            hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_, H_), dim3(256), 0, stream_, dYt_, B_,
+            hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_ / 4, H_), dim3(256), 0, stream_, dYt_, B_,
                                out_pad_, g_ + mo.boff, P_);
             MLP_TRY(hipGetLastError());
             mlp::GemmArgs d = base_args();
@@ -1066,7 +1106,7 @@ class MlpModel {
             const auto &mi = mats_[l];
             weight_grad(mi, dZt_[cur], l == 0 ? Xt_ : Ht_[l - 1],
                         l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
-            hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
+            hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_ / 4, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
                                g_ + mi.boff, P_);
             MLP_TRY(hipGetLastError());
             if (l > 0) {
@@ -1107,7 +1147,12 @@ class MlpModel {
             a.eps = cfg_.eps;
             a.wd = cfg_.weight_decay;
             a.step = step_;
-            hipLaunchKernelGGL(mlp::adam_update, dim3(512, H_), dim3(256), 0, stream_, a);
+            int tiles = 0;
+            for (const auto &mi : mats_) tiles += (mi.out / mlp::kAdamTile) * (mi.in / mlp::kAdamTile);
+            hipLaunchKernelGGL(mlp::adam_matrices, dim3(tiles, H_), dim3(256), 0, stream_, a);
+            MLP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(mlp::adam_biases, dim3(static_cast<unsigned>(mats_.size()), H_),
+                               dim3(256), 0, stream_, a);
             MLP_TRY(hipGetLastError());
             hipLaunchKernelGGL(mlp::bump_step, dim3(1), dim3(1), 0, stream_, step_);
             MLP_TRY(hipGetLastError());
