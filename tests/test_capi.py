@@ -179,3 +179,21 @@ def test_angle_profiles_are_checked_before_any_device_work():
     plan = engine.C.c_void_p()
     assert L.tmc_batched_create(4, engine.C.byref(o), engine.C.byref(plan)) == engine.TMC_ERR_CONFIG
     assert "single-run options" in L.tmc_last_error().decode()
+
+
+def test_time_and_aggregation_options_are_checked_before_any_device_work():
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    bad = [(dict(time=(0, 0.01)), "nt >= 1"), (dict(time=(10, 0.0)), "dt"),
+           (dict(time=(10, float("nan"))), "dt"),
+           (dict(time=(10, 0.01), aggregate=True), "aggregation"),
+           (dict(time=(10, 0.01), profiles=False), "no-profiles"),
+           (dict(aggregate=True, profiles=False), "no-profiles")]
+    for kw, msg in bad:
+        with pytest.raises(engine.ConfigError, match=msg):
+            engine.run(spec, photons=100, batches=1, **kw)
+    L = engine.load()
+    o = engine.TmcBatchedOpts(seed=1, batches_per_stack=1, dz=0.01, dr=0.01, nz=10, nr=10,
+                              flags=engine.FLAG_TIME)
+    plan = engine.C.c_void_p()
+    assert L.tmc_batched_create(4, engine.C.byref(o), engine.C.byref(plan)) == engine.TMC_ERR_CONFIG
+    assert "single-run options" in L.tmc_last_error().decode()
