@@ -96,9 +96,11 @@ def test_every_scalar_and_bin_within_3_sigma_of_float64_oracle(gpu, case):
     """Same input, same seed: per batch the GPU and the float64 oracle follow the same photons,
     so every bin must sit well inside 3 combined standard errors."""
     spec = load_spec(case)
-    n, B = (200_000, 20) if case != "three_layer_c2" else (60_000, 20)
+    # the three-layer workload at 1e6 photons (4.2e8 events: ~3 s of the oracle on 16 threads)
+    n, B = {"v3_three_layer": (200_000, 20), "slab_c1": (1_000_000, 20),
+            "three_layer_c2": (1_000_000, 20)}[case]
     g = gpu.run(spec, photons=n, batches=B, seed=21)
-    o = oracle.run(spec, photons=n, batches=B, seed=21, precision="float64")
+    o = oracle.run(spec, photons=n, batches=B, seed=21, precision="float64", threads=16)
     ph = [b["photons"] for b in o["batches"]]
     for key in ("rd", "tt"):
         eo = stats.batch_estimate([b[key] for b in o["batches"]], ph)
@@ -127,6 +129,34 @@ def test_independent_seeds_agree_statistically(gpu):
         z = stats.profile_z(mg, sg, mo, so)[keep]
         assert np.mean(np.abs(z) > 3) <= 0.05, gk
         assert np.sum(z ** 2) < len(z) + 5 * math.sqrt(2 * len(z)), gk
+
+
+def test_full_three_layer_workload_against_float64_engine(gpu):
+    """The benchmark's own configuration at full size: 1e8 photons on the GPU (seed 1, 100
+    batches, 200 x 200 grid) against the float64 engine at 4e6 photons (seed 2, independent).
+    Every scalar (R_d, T_t, each layer's absorption) lies within 3 combined standard errors; the
+    A_z, R_r and T_r bins' z-scores behave like standard normals."""
+    spec = load_spec("three_layer_c2")
+    g = gpu.run(spec, photons=100_000_000, batches=100, seed=1)
+    o = oracle.run(spec, photons=4_000_000, batches=40, seed=2, precision="float64", threads=16)
+    ph_g = [b["photons"] for b in g["batches"]]
+    ph_o = [b["photons"] for b in o["batches"]]
+    for key in ("rd", "tt"):
+        eg = stats.batch_estimate([b[key] for b in g["batches"]], ph_g)
+        eo = stats.batch_estimate([b[key] for b in o["batches"]], ph_o)
+        assert stats.compare(eg.mean, eg.se, eo.mean, eo.se).passed, key
+    for layer in range(3):
+        eg = stats.batch_estimate([b["absorbed"][layer] for b in g["batches"]], ph_g)
+        eo = stats.batch_estimate([b["absorbed"][layer] for b in o["batches"]], ph_o)
+        assert stats.compare(eg.mean, eg.se, eo.mean, eo.se).passed, layer
+    for gk in ("az_b", "rr_b", "tr_b"):
+        mg, sg = stats.profile_estimate(g[gk].astype(np.float64) * 2.0 ** -32, ph_g)
+        mo, so = stats.profile_estimate(o[gk], ph_o)
+        keep = (sg > 0) & (so > 0)
+        z = stats.profile_z(mg, sg, mo, so)[keep]
+        assert len(z) > 50, gk
+        assert np.mean(np.abs(z) > 3) <= 0.02, (gk, float(np.mean(np.abs(z) > 3)))
+        assert np.sum(z ** 2) < len(z) + 5 * math.sqrt(2 * len(z)), (gk, float(np.sum(z ** 2)), len(z))
 
 
 def test_weight_one_exits_are_exact(gpu):
