@@ -34,6 +34,11 @@
 //   (argument in revolutions, so the azimuth 2*pi*xi needs no multiply), v_sqrt, v_rsq, v_rcp.
 //   The file is compiled without automatic FMA contraction and every fused multiply-add is
 //   written out, so that all instantiations of the kernel follow bit-identical photon paths.
+// * Variants are template parameters, so an option that is off costs the step loop nothing:
+//   kProfiles (A_z, R_r, T_r; off for the reference's no-profiles diagnostic), kArz (the A(r,z)
+//   tile), kAgg (lane aggregation of the A_z and A(r,z) adds, TMC_FLAG_AGGREGATE) and kTime
+//   (time-of-flight profiles, TMC_FLAG_TIME). The exit-angle profiles are a run-time switch on
+//   the escape path, which runs once per photon.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
