@@ -600,22 +600,34 @@ __global__ void load_inputs(const float *xf, int rows, int in, int in_pad, int B
 
 // Ensemble mean of every output and the spread (standard deviation over heads) of the first
 // n_spread outputs, for rows [0, rows).
-__global__ void ensemble_reduce(const float *Yp, int heads, int B, int out_pad, int out, int n_spread,
-                                int rows, float *mean, float *spread) {
-    const int m = static_cast<int>(blockIdx.x);
+// Ensemble mean of every output and spread of the first n_spread (<= 4), per query row m < rows:
+// four rows per block, 64 threads per row, four consecutive outputs per thread (16-byte loads;
+// out_pad is a multiple of 4 at least out rounded up to 4).
+__global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int heads, int B, int out_pad,
+                                                       int out, int n_spread, int rows, float *mean,
+                                                       float *spread) {
+    const int m = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     if (m >= rows) return;
-    for (int j = static_cast<int>(threadIdx.x); j < out; j += static_cast<int>(blockDim.x)) {
-        float s = 0.0f;
-        for (int h = 0; h < heads; ++h) s += Yp[(static_cast<long long>(h) * B + m) * out_pad + j];
-        const float mu = s / heads;
-        mean[static_cast<long long>(m) * out + j] = mu;
-        if (j < n_spread) {  // two passes: no cancellation when the heads nearly agree
-            float ss = 0.0f;
+    for (int c = static_cast<int>(threadIdx.x & 63) * 4; c < out; c += 256) {
+        floatx4 s = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int h = 0; h < heads; ++h)
+            s += *reinterpret_cast<const floatx4 *>(Yp + (static_cast<long long>(h) * B + m) * out_pad + c);
+        const floatx4 mu = s / static_cast<float>(heads);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < out) mean[static_cast<long long>(m) * out + c + q] = mu[q];
+        if (c < n_spread) {  // two passes: no cancellation when the heads nearly agree
+            floatx4 ss = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             for (int h = 0; h < heads; ++h) {
-                const float d = Yp[(static_cast<long long>(h) * B + m) * out_pad + j] - mu;
+                const floatx4 d =
+                    *reinterpret_cast<const floatx4 *>(Yp + (static_cast<long long>(h) * B + m) * out_pad + c) - mu;
                 ss += d * d;
             }
-            spread[static_cast<long long>(m) * n_spread + j] = heads > 1 ? __builtin_sqrtf(ss / (heads - 1)) : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c + q < n_spread)
+                    spread[static_cast<long long>(m) * n_spread + c + q] =
+                        heads > 1 ? __builtin_sqrtf(ss[q] / (heads - 1)) : 0.0f;
         }
     }
 }
@@ -839,7 +851,7 @@ class MlpModel {
             hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (in_pad_ / 8) + 255) / 256), dim3(256), 0,
                                stream_, xq_, rows, cfg_.inputs, in_pad_, Bq_, Xq_);
             forward(Xq_, Bq_, Hq_, nullptr, nullptr, true);
-            hipLaunchKernelGGL(mlp::ensemble_reduce, dim3(rows), dim3(256), 0, stream_, Yp_, H_, Bq_,
+            hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             MLP_TRY(hipGetLastError());
             MLP_TRY(hipEventRecord(e1_, stream_));
