@@ -49,7 +49,7 @@ enum {
 #define TMC_FLAG_AGGREGATE 16u    /* combine a wave's equal-bin A_z and A(r,z) adds (ballot + DPP
                                    * sum) before the LDS atomics; results are identical */
 #define TMC_FLAG_TIME 32u         /* also tally R_d(t), T_t(t) into tmc_out.rt/tt_t (tmc_opts.nt,
-                                   * dt); not with TMC_FLAG_AGGREGATE */
+                                   * dt) */
 
 typedef struct tmc_layer {
     double n;   /* refractive index */

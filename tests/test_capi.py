@@ -185,7 +185,6 @@ def test_time_and_aggregation_options_are_checked_before_any_device_work():
     (spec,) = engine.parse_mci(THREE_LAYER)
     bad = [(dict(time=(0, 0.01)), "nt >= 1"), (dict(time=(10, 0.0)), "dt"),
            (dict(time=(10, float("nan"))), "dt"),
-           (dict(time=(10, 0.01), aggregate=True), "aggregation"),
            (dict(time=(10, 0.01), profiles=False), "no-profiles"),
            (dict(aggregate=True, profiles=False), "no-profiles")]
     for kw, msg in bad:

@@ -15,13 +15,14 @@ from conftest import load_spec
 
 pytestmark = pytest.mark.gpu
 
-FIXED = ("rd", "tt", "absorbed", "az", "rr", "tr", "arz")
+FIXED = ("rd", "tt", "absorbed", "az", "rr", "tr", "arz", "ra", "ta", "rra", "tra", "rt", "tt_t")
 
 
 def same(a, b):
     for key in FIXED:
-        assert np.array_equal(a["fixed"][key], b["fixed"][key]), key
-    for key in ("rd_a", "t_a", "rd_ra", "t_ra"):
+        if key in a["fixed"]:
+            assert np.array_equal(a["fixed"][key], b["fixed"][key]), key
+    for key in ("rd_a", "t_a", "rd_ra", "t_ra", "rd_t", "t_t"):
         if key in a:
             assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), key
     for key in ("events", "boundary_steps", "anomalies"):
@@ -41,6 +42,16 @@ def test_aggregated_tallies_equal_plain(gpu, name, grid):
     agg = engine.run(spec, aggregate=True, **kw)
     same(plain, agg)
     assert agg["fixed"]["arz"].sum() > 0
+
+
+@pytest.mark.parametrize("arz", [True, False])
+def test_aggregated_with_time_profiles(gpu, arz):
+    spec = dict(load_spec("v3_three_layer"), na=4)
+    kw = dict(photons=200_000, batches=8, seed=23, arz=arz, angles=True, time=(40, 0.005))
+    plain = engine.run(spec, **kw)
+    agg = engine.run(spec, aggregate=True, **kw)
+    same(plain, agg)
+    assert plain["fixed"]["rt"].sum() > 0 and plain["fixed"]["tt_t"].sum() > 0
 
 
 def test_aggregated_profiles_without_arz(gpu):

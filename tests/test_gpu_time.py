@@ -94,8 +94,6 @@ def test_time_options_are_checked(gpu):
         engine.run(spec, photons=1000, batches=2, time=(0, 0.01))
     with pytest.raises(engine.ConfigError, match="dt"):
         engine.run(spec, photons=1000, batches=2, time=(10, -1.0))
-    with pytest.raises(engine.ConfigError, match="aggregation"):
-        engine.run(spec, photons=1000, batches=2, time=(10, 0.01), aggregate=True)
 
 
 def test_cli_time(gpu, tmp_path):

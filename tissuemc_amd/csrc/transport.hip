@@ -36,9 +36,11 @@
 //   written out, so that all instantiations of the kernel follow bit-identical photon paths.
 // * Variants are template parameters, so an option that is off costs the step loop nothing:
 //   kProfiles (A_z, R_r, T_r; off for the reference's no-profiles diagnostic), kArz (the A(r,z)
-//   tile), kAgg (lane aggregation of the A_z and A(r,z) adds, TMC_FLAG_AGGREGATE) and kTime
-//   (time-of-flight profiles, TMC_FLAG_TIME). The exit-angle profiles are a run-time switch on
-//   the escape path, which runs once per photon.
+//   tile), kAgg (lane aggregation of the A_z and A(r,z) adds, TMC_FLAG_AGGREGATE) and kExit
+//   (the exit-angle and time-of-flight profiles, TMC_FLAG_ANGLES and TMC_FLAG_TIME, which are
+//   then each a run-time switch on the escape path). Even a branch that is never taken costs the
+//   step loop registers: its kernel arguments stay live across the loop, and past the scalar
+//   register file they are spilled to and reloaded from vector lanes on the hot path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -141,6 +143,10 @@ struct KernelArgs {
     int32_t slot_words, wave_words;    // u64 words: the slot, and slot + layer table, per wave
                                        // (the table starts at the even word after the slot)
     int32_t tile_z, tile_r, tile_off;  // A(r,z) LDS tile: [tile_z x tile_r] u32 at word tile_off
+    // Exit-angle and time-of-flight bins of the workgroup, [ra na][ta na][rt nt][ttt nt] u64 at
+    // word 0 (totals over all batches, flushed once at the end); the wave regions follow them
+    int32_t ext_words;
+    int32_t lds_words;  // u64 words of LDS the workgroup uses (cleared at start)
     uint32_t chunk;
     uint32_t step_limit;
 };
@@ -366,20 +372,19 @@ __device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, 
     }
 }
 
-template <bool kProfiles, bool kArz, bool kAgg, bool kTime>
+template <bool kProfiles, bool kArz, bool kAgg, bool kExit>
 __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     extern __shared__ uint64_t lds[];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wid = static_cast<int>(threadIdx.x >> 6);
     const int waves = static_cast<int>(blockDim.x >> 6);
 
-    // Clear the wave regions and the A(r,z) tile.
-    const int words = waves * a.wave_words + (kArz ? (a.tile_z * a.tile_r + 1) / 2 : 0);
-    for (int i = static_cast<int>(threadIdx.x); i < words; i += static_cast<int>(blockDim.x))
+    // Clear the wave regions, the A(r,z) tile and the exit-angle / time bins.
+    for (int i = static_cast<int>(threadIdx.x); i < a.lds_words; i += static_cast<int>(blockDim.x))
         lds[i] = 0;
     __syncthreads();
 
-    uint64_t *slot = lds + wid * a.wave_words;
+    uint64_t *slot = lds + a.ext_words + wid * a.wave_words;
     float *ltab = reinterpret_cast<float *>(slot + ((a.slot_words + 1) & ~1));
     const uint32_t ltab_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_float *)ltab));
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
     uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
     uint64_t acc = 0;
-    float opl = 0;  // kTime: optical path length so far (cm): sum of n s over the hops
+    float opl = 0;  // kExit: optical path length so far (cm): sum of n s over the hops
 
     while (have) {
         for (;;) {
@@ -445,7 +450,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 uz = start ? 1.0f : uz;
                 w = start ? wu.w0 : w;
                 sleft = start ? 0.0f : sleft;
-                if (kTime) opl = start ? 0.0f : opl;
+                if (kExit) opl = start ? 0.0f : opl;
                 layer = start ? 0 : layer;
                 nstep = start ? 0u : nstep;
                 acc = start ? 0ull : acc;
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             z = __builtin_fmaf(s, uz, z);
             // time of flight: the hop's optical path, n s, added as a product then a sum (the
             // oracle's float32 arithmetic, so both bin the same photons alike)
-            if (kTime) opl = opl + s * P.f[kN];
+            if (kExit) opl = opl + s * P.f[kN];
 
             bool died = false;
             // With kAgg the event's A_z and A(r,z)-tile scores are made after the branch, where
@@ -573,21 +578,21 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                 uint64_t *row = (exit_top ? a.rr : a.tr) +
                                                 static_cast<uint64_t>(wu.prow) * a.nr;
                                 hbm_add(row + ir, qw);
-                                if (a.na > 0) {
+                                if (kExit && a.na > 0) {
                                     // MCML's exit angle: from the normal, acos of the
                                     // escaping |uz| (= ca2), in na bins of pi / (2 na)
                                     const int ia = bin_of(acosf(ca2), a.inv_da, last_a);
-                                    hbm_add((exit_top ? a.ra : a.ta) + ia, qw);
+                                    lds_add(lds + (exit_top ? 0 : a.na) + ia, qw);
                                     hbm_add((exit_top ? a.rra : a.tra) +
                                                 __umul24(static_cast<uint32_t>(ir),
                                                          static_cast<uint32_t>(a.na)) +
                                                 ia,
                                             qw);
                                 }
-                                if (kTime) {
+                                if (kExit && a.nt > 0) {
                                     // time of flight opl / c, in nt bins of dt, clamped
                                     const int it = bin_of(opl, a.inv_dopl, last_t);
-                                    hbm_add((exit_top ? a.rt : a.ttt) + it, qw);
+                                    lds_add(lds + 2 * a.na + (exit_top ? 0 : a.nt) + it, qw);
                                 }
                             }
                         }
@@ -736,6 +741,19 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             const uint64_t v = tile[i];
             if (v != 0)
                 hbm_add(a.arz + static_cast<uint64_t>(i / a.tile_r) * a.nr + (i % a.tile_r), v);
+        }
+    }
+    if (kProfiles && a.ext_words > 0) {
+        __syncthreads();
+        for (int i = static_cast<int>(threadIdx.x); i < a.ext_words; i += static_cast<int>(blockDim.x)) {
+            const uint64_t v = lds[i];
+            if (v == 0) continue;
+            const int na2 = 2 * a.na;
+            uint64_t *dst = i < a.na ? a.ra + i
+                          : i < na2 ? a.ta + (i - a.na)
+                          : i < na2 + a.nt ? a.rt + (i - na2)
+                                           : a.ttt + (i - na2 - a.nt);
+            hbm_add(dst, v);
         }
     }
 }
@@ -891,7 +909,7 @@ class Resident {
         profiles_ = p.profiles;
         arz_ = p.arz;
         agg_ = p.aggregate;
-        time_ = p.nt > 0;
+        exit_ = p.na > 0 || p.nt > 0;
         a_.slot_words = kSlotAbs + p.lmax + (p.profiles ? p.nz : 0);
         a_.wave_words = ((a_.slot_words + 1) & ~1) + p.lmax * kLayerFields / 2;
 
@@ -906,9 +924,18 @@ class Resident {
                               "the profile grid is too large for one wave's LDS tally (" +
                                   std::to_string(wave_bytes(1)) + " bytes; the device has " +
                                   std::to_string(cap) + " per workgroup)");
+        // The exit-angle and time bins of the workgroup (a few KB) sit in LDS ahead of the wave
+        // slots: every escape adds to one of a few hot bins, which HBM atomics serialise.
+        const size_t ext_words = 2 * (static_cast<size_t>(p.na) + static_cast<size_t>(p.nt));
+        const size_t ext_bytes = ext_words * 8;
+        while (wpg > 1 && wave_bytes(wpg) + ext_bytes > cap) --wpg;
+        if (wave_bytes(wpg) + ext_bytes > cap)
+            throw EngineError(TMC_ERR_CONFIG,
+                              "the exit-angle and time bins (2 (na + nt) = " + std::to_string(ext_words) +
+                                  " counters) do not fit in LDS beside one wave's tally");
         int tile_z = 0, tile_r = 0;
         if (p.arz) {
-            const size_t free_bins = (cap - wave_bytes(wpg)) / 4;  // u32 counters
+            const size_t free_bins = (cap - wave_bytes(wpg) - ext_bytes) / 4;  // u32 counters
             if (t.arz_tile_z >= 0 && t.arz_tile_r >= 0) {
                 tile_z = std::min(t.arz_tile_z, p.nz);
                 tile_r = std::min(t.arz_tile_r, p.nr);
@@ -928,8 +955,10 @@ class Resident {
         }
         a_.tile_z = tile_z;
         a_.tile_r = tile_r;
-        a_.tile_off = wpg * a_.wave_words;
-        lds_bytes_ = (static_cast<size_t>(a_.tile_off) + (static_cast<size_t>(tile_z) * tile_r + 1) / 2) * 8;
+        a_.ext_words = static_cast<int32_t>(ext_words);
+        a_.tile_off = a_.ext_words + wpg * a_.wave_words;
+        a_.lds_words = a_.tile_off + static_cast<int32_t>((static_cast<size_t>(tile_z) * tile_r + 1) / 2);
+        lds_bytes_ = static_cast<size_t>(a_.lds_words) * 8;
         wpg_ = wpg;
         groups_ = t.groups > 0 ? t.groups : dev.cus;
         const uint64_t waves_total = static_cast<uint64_t>(groups_) * wpg;
@@ -1100,9 +1129,12 @@ class Resident {
     }
     const void *kernel() const {
         if (!profiles_) return kernel_ptr<false, false, false, false>();
-        if (time_) return arz_ ? kernel_ptr<true, true, false, true>() : kernel_ptr<true, false, false, true>();
-        if (agg_) return arz_ ? kernel_ptr<true, true, true, false>() : kernel_ptr<true, false, true, false>();
-        return arz_ ? kernel_ptr<true, true, false, false>() : kernel_ptr<true, false, false, false>();
+        static const void *const with_profiles[8] = {
+            kernel_ptr<true, false, false, false>(), kernel_ptr<true, false, false, true>(),
+            kernel_ptr<true, false, true, false>(),  kernel_ptr<true, false, true, true>(),
+            kernel_ptr<true, true, false, false>(),  kernel_ptr<true, true, false, true>(),
+            kernel_ptr<true, true, true, false>(),   kernel_ptr<true, true, true, true>()};
+        return with_profiles[(arz_ ? 4 : 0) + (agg_ ? 2 : 0) + (exit_ ? 1 : 0)];
     }
     void enqueue_clear() {
         const unsigned grid = static_cast<unsigned>(std::min<size_t>((out_words_ + 255) / 256, 4096));
@@ -1112,26 +1144,13 @@ class Resident {
     }
     void enqueue_transport() {
         const dim3 grid(static_cast<unsigned>(groups_)), block(static_cast<unsigned>(wpg_ * 64));
-        if (!profiles_)
-            hipLaunchKernelGGL((transport_kernel<false, false, false, false>), grid, block, lds_bytes_, stream_, a_);
-        else if (time_ && arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true, false, true>), grid, block, lds_bytes_, stream_, a_);
-        else if (time_)
-            hipLaunchKernelGGL((transport_kernel<true, false, false, true>), grid, block, lds_bytes_, stream_, a_);
-        else if (agg_ && arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true, true, false>), grid, block, lds_bytes_, stream_, a_);
-        else if (agg_)
-            hipLaunchKernelGGL((transport_kernel<true, false, true, false>), grid, block, lds_bytes_, stream_, a_);
-        else if (arz_)
-            hipLaunchKernelGGL((transport_kernel<true, true, false, false>), grid, block, lds_bytes_, stream_, a_);
-        else
-            hipLaunchKernelGGL((transport_kernel<true, false, false, false>), grid, block, lds_bytes_, stream_, a_);
-        HIP_TRY(hipGetLastError());
+        void *args[] = {&a_};
+        HIP_TRY(hipLaunchKernel(kernel(), grid, block, args, lds_bytes_, stream_));
     }
 
     DeviceInfo dev_;
     KernelArgs a_;
-    bool profiles_ = true, arz_ = false, agg_ = false, time_ = false;
+    bool profiles_ = true, arz_ = false, agg_ = false, exit_ = false;
     int wpg_ = 16, groups_ = 0;
     size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
     void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
@@ -1165,9 +1184,6 @@ tmc_spec checked_single(const tmc_spec &spec_in, const tmc_opts &opt) {
         if (!profiles)
             throw EngineError(TMC_ERR_CONFIG,
                               "the time-resolved profiles cannot be combined with no-profiles");
-        if ((opt.flags & TMC_FLAG_AGGREGATE) != 0)
-            throw EngineError(TMC_ERR_CONFIG,
-                              "the time-resolved profiles cannot be combined with lane aggregation");
         if (opt.nt < 1) throw EngineError(TMC_ERR_CONFIG, "the time-resolved profiles need nt >= 1");
         if (!(opt.dt > 0.0) || !std::isfinite(opt.dt))
             throw EngineError(TMC_ERR_CONFIG, "the time bin width dt must be > 0 (ns)");

@@ -1,7 +1,7 @@
 """Hardware-counter profile of the transport kernel with rocprofv3, and a side-by-side digest.
 
     python -m tissuemc_amd.profile run --output results/c2.json [--mci FILE] [--photons N]
-        [--batches B] [--arz] [--no-profiles] [--angles] [--aggregate]
+        [--batches B] [--arz] [--no-profiles] [--angles] [--aggregate] [--time NT DT]
     python -m tissuemc_amd.profile digest results/c2.json results/c2-no-profiles.json
 
 The counterpart of the reference's Nsight Compute tool (python/tissuemc_analysis/ncu_profile.py:
@@ -179,6 +179,7 @@ def main(argv: list[str] | None = None) -> int:
     r.add_argument("--arz", action="store_true")
     r.add_argument("--angles", action="store_true")
     r.add_argument("--aggregate", action="store_true")
+    r.add_argument("--time", nargs=2, metavar=("NT", "DT"), help="time-of-flight bins and width (ns)")
     r.add_argument("--no-profiles", action="store_true")
     r.add_argument("--kernel", default="transport_kernel")
     r.add_argument("--fetch-scale", type=float, default=2.0)
@@ -200,6 +201,8 @@ def main(argv: list[str] | None = None) -> int:
     program = [str(CLI), "run", str(args.mci.resolve()), "--photons", str(args.photons),
                "--batches", str(args.batches), "--seed", str(args.seed)]
     program += ["--arz"] * args.arz + ["--angles"] * args.angles + ["--aggregate"] * args.aggregate
+    if args.time:
+        program += ["--time", *args.time]
     program += ["--no-profiles"] * args.no_profiles
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         res = profile(program, args.kernel, Path(tmp), args.fetch_scale, args.write_scale)
