@@ -76,8 +76,7 @@ __device__ __forceinline__ float gelu_s(float z) {
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
 // d/dz: s + z (1 - tanh^2 u) u' / 2 = s + 2 z s (1 - s) sqrt(2/pi) (1 + 3 a z^2)
-__device__ __forceinline__ float gelu_grad(float z) {
-    const float s = gelu_s(z);
+__device__ __forceinline__ float gelu_grad_s(float z, float s) {
     const float du = __builtin_fmaf(3.0f * kGeluA * 2.0f * kGeluC, z * z, 2.0f * kGeluC);
     return __builtin_fmaf(z * s * (1.0f - s), du, s);
 }
@@ -95,15 +94,27 @@ struct GemmArgs {
     // epilogue operands (head strides in elements)
     const float *bias;
     long long sBias;
-    float *Cf;  // fp32 output [M x N] (Z for kFwdHidden, Y for kFwdOut, dW partials for kBwdWeight)
+    float *Cf;  // fp32 output [M x N] (Y for kFwdOut, dW partials for kBwdWeight)
     long long sCf;
     long long sSplit;  // kBwdWeight: stride between split partials
     bf16 *Cb;   // bf16 output [M x N] (H or dZ)
     bf16 *Ct;   // bf16 transposed output [N x M] (Ht or dZt)
     long long sCb;  // head stride of Cb and Ct (M * N)
-    const float *Zin;  // kBwdData: the pre-activation of this layer [M x N]
-    long long sZin;
+    bf16 *Gd;         // kFwdHidden (training): GELU'(Z) as bf16, M x N in fragment order
+    const bf16 *Gin;  // kBwdData: the GELU'(Z) of this layer, as kFwdHidden wrote it
 };
+
+// Fragment order: the layout in which the training forward hands GELU'(Z) to the backward data
+// product of the same layer. Both compute the same M x N output on the same 128 x 128 tiles and
+// wave and fragment split, so a lane's four values (rows mb .. mb + 3 of column n) are stored
+// where the other kernel's lane will read them: 8 contiguous bytes per lane, 512 per wave and
+// fragment, instead of 4-byte values strided by rows. Offset in elements: tile (row-major over
+// the N / 128 column tiles), then wave, fragment (i, j), lane.
+__device__ __forceinline__ long long frag_off(int mblk, int nblk, int N, int wave, int i, int j,
+                                              int lane) {
+    const long long t = static_cast<long long>(mblk >> 7) * (N >> 7) + (nblk >> 7);
+    return (((t * 4 + wave) * 16 + i * 4 + j) * 64 + lane) * 4;
+}
 
 // C[m, n] = sum_k A[m, k] B[n, k] on a 128 x 128 tile per workgroup: four waves in a 2 x 2
 // arrangement, each accumulating 64 x 64 as 4 x 4 MFMA tiles of 16 x 16.
@@ -146,9 +157,19 @@ typedef __attribute__((address_space(3))) void lds_void;
 // fence would) nor moves LDS accesses across it.
 template <int N>
 __device__ __forceinline__ void stage_sync() {
-    if constexpr (N == 8) asm volatile("s_waitcnt vmcnt(8) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    else if constexpr (N == 4) asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    static_assert(N >= 0 && N <= 63, "vmcnt is a 6-bit count");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+// stage_sync for `ahead` stages (four loads each) still allowed in flight, 0 <= ahead <= kMax
+// (wave-uniform)
+template <int kMax>
+__device__ __forceinline__ void stage_sync_ahead(int ahead) {
+    if constexpr (kMax == 0) {
+        stage_sync<0>();
+    } else {
+        if (ahead >= kMax) stage_sync<4 * kMax>();
+        else stage_sync_ahead<kMax - 1>(ahead);
+    }
 }
 
 template <int kEpi>
@@ -219,9 +240,7 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     for (int kt = 0; kt < nk; ++kt) {
         // stages issued after kt so far: min(kStages - 2, nk - 1 - kt), four loads each
         const int ahead = nk - 1 - kt < kStages - 2 ? nk - 1 - kt : kStages - 2;
-        if (ahead >= 2) stage_sync<8>();
-        else if (ahead == 1) stage_sync<4>();
-        else stage_sync<0>();
+        stage_sync_ahead<kStages - 2>(ahead);
         if (kt + kStages - 1 < nk) TMC_GEMM_ISSUE(kt + kStages - 1);
         const bf16 *As = smem + (kt % kStages) * kStageElems, *Bs = As + 128 * kBK;
         bf16x8 fa[4], fb[4];
@@ -276,22 +295,28 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int col = wn + 16 * j + r, rb = wm + 16 * i + (lane >> 4) * 4;
-                const int n = nblk + col, mb = mblk + rb;
+                const int n = nblk + col;
+                const long long fo = h * g.sCb + frag_off(mblk, nblk, g.N, wave, i, j, lane);
                 float v[4];
                 if (kEpi == kFwdHidden) {
-                    float *Zp = g.Cf ? g.Cf + h * g.sCf : nullptr;
                     const float b = g.bias[h * g.sBias + n];
+                    float gd[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const float zv = acc[i][j][q] + b;
-                        if (Zp) Zp[static_cast<long long>(mb + q) * g.N + n] = zv;
-                        v[q] = gelu(zv);
+                        const float s = gelu_s(zv);
+                        v[q] = zv * s;
+                        gd[q] = gelu_grad_s(zv, s);
                     }
+                    if (g.Gd)
+                        *reinterpret_cast<uint2 *>(g.Gd + fo) =
+                            uint2{pack_bf16x2(gd[0], gd[1]), pack_bf16x2(gd[2], gd[3])};
                 } else {  // kBwdData
-                    const float *Zp = g.Zin + h * g.sZin;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        v[q] = acc[i][j][q] * gelu_grad(Zp[static_cast<long long>(mb + q) * g.N + n]);
+                    const uint2 gg = *reinterpret_cast<const uint2 *>(g.Gin + fo);
+                    v[0] = acc[i][j][0] * from_bf16(static_cast<bf16>(gg.x));
+                    v[1] = acc[i][j][1] * from_bf16(static_cast<bf16>(gg.x >> 16));
+                    v[2] = acc[i][j][2] * from_bf16(static_cast<bf16>(gg.y));
+                    v[3] = acc[i][j][3] * from_bf16(static_cast<bf16>(gg.y >> 16));
                 }
                 const uint32_t p01 = pack_bf16x2(v[0], v[1]), p23 = pack_bf16x2(v[2], v[3]);
                 T[rb * kPitch + col] = static_cast<bf16>(p01);
@@ -349,17 +374,24 @@ __global__ void reduce_splits(const float *part, int splits, long long count, lo
 }
 
 // Batch assembly for a block of 16 rows of head h: row m takes dataset sample idx (Philox of
-// (step, m, h) mod n); writes the standardized features padded to in_pad as bf16 X [B x in_pad]
-// and, through an LDS tile, Xt [in_pad x B], and the targets padded to out_pad as fp32
-// Y [B x out_pad]. Consecutive threads take consecutive columns, so every access is coalesced.
+// (step, m, h) mod n); writes the standardized features as bf16 X [B x in_pad] and, through an
+// LDS tile, Xt [in_pad x B], and the targets padded to out_pad as fp32 Y [B x out_pad].
+// Consecutive threads take consecutive columns, so every access is coalesced. Only the first
+// in_used columns of X (rows of Xt) are written, zero past `in`: the rest stay as allocated
+// (zero). The rows are random, so each read is a separate trip to memory: a thread issues all of
+// its reads before its first store, and waits for them once rather than once per row. in_used is
+// a multiple of 32 and at most in_pad.
 constexpr int kGatherRows = 16;
-__global__ __launch_bounds__(256) void gather_batch(const float *xd, const float *yd, int64_t n,
-                                                    int in, int out, int in_pad, int out_pad, int B,
+__global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd,
+                                                    const float *__restrict__ yd, int64_t n,
+                                                    int in, int out, int in_used, int in_pad,
+                                                    int out_pad, int B,
                                                     const uint64_t *step, uint32_t key0,
-                                                    uint32_t key1, bf16 *X, bf16 *Xt, float *Y,
+                                                    uint32_t key1, bf16 *__restrict__ X,
+                                                    bf16 *__restrict__ Xt, float *__restrict__ Y,
                                                     const int64_t *fixed_rows) {
     __shared__ int64_t rows[kGatherRows];
-    __shared__ bf16 tile[128][kGatherRows + 2];  // [column][row] of one 128-column slab
+    __shared__ bf16 tile[32][kGatherRows + 2];  // [column][row] of one 32-column slab
     const int tid = static_cast<int>(threadIdx.x);
     const int m0 = static_cast<int>(blockIdx.x) * kGatherRows;
     const int h = static_cast<int>(blockIdx.y);
@@ -379,32 +411,39 @@ __global__ __launch_bounds__(256) void gather_batch(const float *xd, const float
     bf16 *Xh = X + static_cast<long long>(h) * B * in_pad;
     bf16 *Xth = Xt + static_cast<long long>(h) * B * in_pad;
     float *Yh = Y + static_cast<long long>(h) * B * out_pad;
-    for (int j0 = 0; j0 < in_pad; j0 += 128) {
-        // 16 rows x 128 columns: thread -> column tid % 128, rows tid / 128 + 2 k
-        const int j = j0 + (tid & 127);
-        for (int i = tid >> 7; i < kGatherRows; i += 2) {
-            const bf16 v = to_bf16(j < in ? xd[rows[i] * in + j] : 0.0f);
-            Xh[static_cast<long long>(m0 + i) * in_pad + j] = v;
-            tile[tid & 127][i] = v;
-        }
+    for (int j0 = 0; j0 < in_used; j0 += 32) {
+        // 16 rows x 32 columns: thread -> column tid % 32, rows tid / 32 and tid / 32 + 8
+        const int j = j0 + (tid & 31), i0 = tid >> 5;
+        const float v0 = j < in ? xd[rows[i0] * in + j] : 0.0f;
+        const float v1 = j < in ? xd[rows[i0 + 8] * in + j] : 0.0f;
+        const bf16 b0 = to_bf16(v0), b1 = to_bf16(v1);
+        Xh[static_cast<long long>(m0 + i0) * in_pad + j] = b0;
+        Xh[static_cast<long long>(m0 + i0 + 8) * in_pad + j] = b1;
+        tile[tid & 31][i0] = b0;
+        tile[tid & 31][i0 + 8] = b1;
         __syncthreads();
-        // Xt rows j0 .. j0 + 127, 16 rows of X each: thread -> (column tid / 16 + 16 k, row tid % 16)
-        for (int c = tid >> 4; c < 128; c += 16)
+        // Xt rows j0 .. j0 + 31, 16 rows of X each: thread -> (column tid / 16 + 16 k, row tid % 16)
+        for (int c = tid >> 4; c < 32; c += 16)
             Xth[static_cast<long long>(j0 + c) * B + m0 + (tid & 15)] = tile[c][tid & 15];
         __syncthreads();
     }
-    for (int e = tid; e < kGatherRows * out_pad; e += 256) {
-        const int i = e / out_pad, j = e % out_pad;
-        Yh[static_cast<long long>(m0 + i) * out_pad + j] = j < out ? yd[rows[i] * out + j] : 0.0f;
+    for (int j = tid; j < out_pad; j += 256) {
+        float v[kGatherRows];
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) v[i] = j < out ? yd[rows[i] * out + j] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) Yh[static_cast<long long>(m0 + i) * out_pad + j] = v[i];
     }
 }
 
 // Weighted squared error over the real outputs on a 64-row x 64-column tile of head h: writes dY
 // (bf16) and, through an LDS tile, its transpose dYt (both coalesced), and the tile's share of
 // the loss into part[h][tile] (summed in a fixed order by loss_reduce).
-__global__ __launch_bounds__(256) void loss_grad(const float *Yp, const float *Yt, const float *w,
-                                                 int B, int out_pad, float inv_count, bf16 *dY,
-                                                 bf16 *dYt, float *part) {
+__global__ __launch_bounds__(256) void loss_grad(const float *__restrict__ Yp,
+                                                 const float *__restrict__ Yt,
+                                                 const float *__restrict__ w, int B, int out_pad,
+                                                 float inv_count, bf16 *__restrict__ dY,
+                                                 bf16 *__restrict__ dYt, float *__restrict__ part) {
     __shared__ bf16 tile[64][66];
     __shared__ float red[256];
     const int tx = static_cast<int>(threadIdx.x & 63u), ty = static_cast<int>(threadIdx.x >> 6);
@@ -412,12 +451,20 @@ __global__ __launch_bounds__(256) void loss_grad(const float *Yp, const float *Y
     const int h = static_cast<int>(blockIdx.z);
     const long long hb = static_cast<long long>(h) * B;
     const float wj = w[j0 + tx];
+    // a thread's 16 rows: every read issued before the first store
+    float d[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const long long e = (hb + m0 + ty + 4 * k) * out_pad + j0 + tx;
+        d[k] = Yp[e] - Yt[e];
+    }
     float acc = 0.0f;
-    for (int i = ty; i < 64; i += 4) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = ty + 4 * k;
         const long long e = (hb + m0 + i) * out_pad + j0 + tx;
-        const float d = Yp[e] - Yt[e];
-        acc += wj * d * d;
-        const bf16 g = to_bf16(2.0f * wj * d * inv_count);
+        acc += wj * d[k] * d[k];
+        const bf16 g = to_bf16(2.0f * wj * d[k] * inv_count);
         dY[e] = g;
         tile[tx][i] = g;
     }
@@ -479,9 +526,12 @@ struct MatInfo {
     long long wt_off, w_off;  // offsets of the bf16 copies in a head's bf16 block
 };
 constexpr int kMaxMats = 8;
+constexpr int kMaxSplits = 8;  // split-K pieces of a weight gradient
 
 struct AdamArgs {
-    float *p, *g, *m, *v;  // [heads x P]
+    float *p, *g, *m, *v;  // [heads x P]; g holds the bias gradients
+    const float *part;     // [heads x splits x P]: split-K partials of the matrix gradients
+    int splits;
     bf16 *wb;              // [heads x Pb] bf16 copies (Wt then W per matrix)
     long long P, Pb;
     int n_mats;
@@ -491,10 +541,11 @@ struct AdamArgs {
     const uint64_t *step;
 };
 
-// Adam (Kingma & Ba 2015) with bias correction for one element; returns the new parameter.
-__device__ __forceinline__ float adam_step(const AdamArgs &a, float *p, const float *g, float *m,
+// Adam (Kingma & Ba 2015) with bias correction for one element of gradient g; returns the new
+// parameter.
+__device__ __forceinline__ float adam_step(const AdamArgs &a, float *p, float g, float *m,
                                            float *v, long long e, float c1, float c2) {
-    const float gr = g[e] + a.wd * p[e];
+    const float gr = g + a.wd * p[e];
     const float mn = a.beta1 * m[e] + (1.0f - a.beta1) * gr;
     const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
     m[e] = mn;
@@ -504,15 +555,30 @@ __device__ __forceinline__ float adam_step(const AdamArgs &a, float *p, const fl
     return pn;
 }
 
-// Adam on the weight matrices, one 64 x 64 tile of one matrix per block (blockIdx.x counts the
-// tiles of all matrices in order, blockIdx.y is the head), and the bf16 copies the products
-// read: W [out x in] row by row like the fp32 parameters, and its transpose [in x out] through
-// an LDS tile, so that every global access is coalesced.
+// Adam on every parameter of one head (blockIdx.y). Blocks [0, tiles) take one 64 x 64 tile of
+// one weight matrix each (the tiles of all matrices in order): the gradient is the sum of the
+// matrix's split-K partials, added in split order as reduce_splits does (so it is the same
+// float), and the block writes the bf16 copies the products read: W [out x in] row by row like
+// the fp32 parameters, and its transpose [in x out] through an LDS tile, so that every global
+// access is coalesced. Block tiles + k updates the bias of matrix k from the gradient block.
 constexpr int kAdamTile = 64;
-__global__ __launch_bounds__(256) void adam_matrices(const AdamArgs a) {
+__global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) {
     __shared__ bf16 tile[kAdamTile][kAdamTile + 2];  // [i][o]
     const int h = static_cast<int>(blockIdx.y);
-    int t = static_cast<int>(blockIdx.x), k = 0;
+    const float tt = static_cast<float>(a.step[0] + 1);
+    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, tt));
+    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, tt));
+    float *p = a.p + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
+    const int tid = static_cast<int>(threadIdx.x);
+    int t = static_cast<int>(blockIdx.x);
+    if (t >= tiles) {
+        const MatInfo mi = a.mats[t - tiles];
+        const float *g = a.g + h * a.P;
+        for (int o = tid; o < mi.out; o += static_cast<int>(blockDim.x))
+            adam_step(a, p, g[mi.boff + o], m, v, mi.boff + o, c1, c2);
+        return;
+    }
+    int k = 0;
     for (; k < a.n_mats; ++k) {
         const int n = (a.mats[k].out / kAdamTile) * (a.mats[k].in / kAdamTile);
         if (t < n) break;
@@ -521,34 +587,33 @@ __global__ __launch_bounds__(256) void adam_matrices(const AdamArgs a) {
     const MatInfo mi = a.mats[k];
     const int tiles_i = mi.in / kAdamTile;
     const int o0 = (t / tiles_i) * kAdamTile, i0 = (t % tiles_i) * kAdamTile;
-    const float tt = static_cast<float>(a.step[0] + 1);
-    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, tt));
-    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, tt));
-    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
+    const float *part = a.part + static_cast<long long>(h) * a.splits * a.P;
     bf16 *wb = a.wb + h * a.Pb;
-    const int tid = static_cast<int>(threadIdx.x);
     const int c = tid & 63;
-    for (int r = tid >> 6; r < kAdamTile; r += 4) {  // row o0 + r, column i0 + c
-        const long long rel = static_cast<long long>(o0 + r) * mi.in + i0 + c;
-        const bf16 b = to_bf16(adam_step(a, p, g, m, v, mi.off + rel, c1, c2));
-        wb[mi.wt_off + rel] = b;
-        tile[c][r] = b;
+    // rows o0 + r, column i0 + c; four rows at a time, so that their partial loads are in flight
+    // together (at most kMaxSplits partials)
+    for (int r0 = tid >> 6; r0 < kAdamTile; r0 += 16) {
+        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int sp = 0; sp < kMaxSplits; ++sp) {
+            if (sp < a.splits) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    g[q] += part[sp * a.P + mi.off + static_cast<long long>(o0 + r0 + 4 * q) * mi.in + i0 + c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = r0 + 4 * q;
+            const long long rel = static_cast<long long>(o0 + r) * mi.in + i0 + c;
+            const bf16 b = to_bf16(adam_step(a, p, g[q], m, v, mi.off + rel, c1, c2));
+            wb[mi.wt_off + rel] = b;
+            tile[c][r] = b;
+        }
     }
     __syncthreads();
     for (int r = tid >> 6; r < kAdamTile; r += 4)  // transposed: row i0 + r, column o0 + c
         wb[mi.w_off + static_cast<long long>(i0 + r) * mi.out + o0 + c] = tile[r][c];
-}
-
-// Adam on the biases: block x is a matrix, block y a head.
-__global__ __launch_bounds__(256) void adam_biases(const AdamArgs a) {
-    const int h = static_cast<int>(blockIdx.y);
-    const MatInfo mi = a.mats[blockIdx.x];
-    const float tt = static_cast<float>(a.step[0] + 1);
-    const float c1 = 1.0f / (1.0f - __builtin_powf(a.beta1, tt));
-    const float c2 = 1.0f / (1.0f - __builtin_powf(a.beta2, tt));
-    float *p = a.p + h * a.P, *g = a.g + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
-    for (int o = static_cast<int>(threadIdx.x); o < mi.out; o += static_cast<int>(blockDim.x))
-        adam_step(a, p, g, m, v, mi.boff + o, c1, c2);
 }
 
 __global__ void bump_step(uint64_t *step) { step[0] += 1; }
@@ -698,12 +763,11 @@ class MlpModel {
         MLP_TRY(hipEventCreate(&e1_));
         p_ = alloc<float>(H_ * P_);
         g_ = alloc<float>(H_ * P_);
-        // split-K of the weight gradients: about 256 batch rows per split, each split a whole
-        // number of 64-row GEMM stages
         // weight gradients split the batch (their K) into pieces of 1024: measured at batch 4096,
         // 1024 beats 256, 512, 2048 and no split (0.447 ms a step against 0.498, 0.458, 0.492,
-        // 0.601): fewer partials to reduce, and still 256 workgroups for a hidden layer's product
-        splits_ = std::max(1, B_ / 1024);
+        // 0.601): fewer partials to reduce, and still 256 workgroups for a hidden layer's product.
+        // Each split is a whole number of GEMM stages; Adam sums at most kMaxSplits partials.
+        splits_ = std::clamp(B_ / 1024, 1, mlp::kMaxSplits);
         while (B_ % (mlp::kBK * splits_) != 0) --splits_;
         part_ = alloc<float>(static_cast<long long>(H_) * splits_ * P_);
         m_ = alloc<float>(H_ * P_);
@@ -722,7 +786,7 @@ class MlpModel {
         dY_ = alloc<bf16>(H_ * B_ * out_pad_);
         dYt_ = alloc<bf16>(H_ * B_ * out_pad_);
         for (int l = 0; l < c.depth; ++l) {
-            Z_.push_back(alloc<float>(H_ * BH));
+            G_.push_back(alloc<bf16>(H_ * BH));
             Hs_.push_back(alloc<bf16>(H_ * BH));
             Ht_.push_back(alloc<bf16>(H_ * BH));
         }
@@ -987,8 +1051,10 @@ class MlpModel {
     }
 
     // dW [out x in] of matrix mi = A (dZt [out x B]) . B (Ht or Xt [in x B])^T, split along the
-    // batch; partials reduced into the gradient block.
-    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht) {
+    // batch; with `reduce` the partials are summed into the gradient block (a training step
+    // leaves them to Adam, which sums them itself).
+    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht,
+                     bool reduce) {
         mlp::GemmArgs g = base_args();
         g.A = dzt;
         g.lda = B_;
@@ -1004,6 +1070,7 @@ class MlpModel {
         g.sCf = static_cast<long long>(splits_) * P_;
         g.sSplit = P_;
         gemm<mlp::kBwdWeight>(g);
+        if (!reduce) return;
         const long long count = static_cast<long long>(mi.out) * mi.in;
         hipLaunchKernelGGL(mlp::reduce_splits, dim3(static_cast<unsigned>(std::min<long long>((count / 4 + 255) / 256, 1024)), H_),
                            dim3(256), 0, stream_, part_ + mi.off, splits_, count,
@@ -1019,7 +1086,7 @@ class MlpModel {
 
     // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
     // alternate between two buffers and neither Z nor Ht is kept.
-    void forward(bf16 *X, int Bn, bf16 *const *Hbuf, float *const *Zbuf, bf16 *const *Htbuf, bool infer) {
+    void forward(bf16 *X, int Bn, bf16 *const *Hbuf, bf16 *const *Gbuf, bf16 *const *Htbuf, bool infer) {
         const long long BH = static_cast<long long>(Bn) * hid_;
         const bf16 *in = X;
         int in_dim = in_pad_;
@@ -1039,8 +1106,7 @@ class MlpModel {
             g.K = l == 0 ? k0_ : in_dim;
             g.bias = p_ + mi.boff;
             g.sBias = P_;
-            g.Cf = infer ? nullptr : Zbuf[l];
-            g.sCf = BH;
+            g.Gd = infer ? nullptr : Gbuf[l];
             g.Cb = infer ? Hbuf[l % 2] : Hbuf[l];
             g.Ct = infer ? nullptr : Htbuf[l];
             g.sCb = BH;
@@ -1075,10 +1141,10 @@ class MlpModel {
         const uint32_t k1 = static_cast<uint32_t>((cfg_.seed >> 32) ^ 0xB5297A4DULL);
         hipLaunchKernelGGL(mlp::gather_batch, dim3(B_ / mlp::kGatherRows, H_), dim3(256), 0, stream_, xd_,
                            yd_, n_data_,
-                           cfg_.inputs, cfg_.outputs, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
+                           cfg_.inputs, cfg_.outputs, k0_, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
                            Yt_, fixed_rows);
         MLP_TRY(hipGetLastError());
-        forward(X_, B_, Hs_.data(), Z_.data(), Ht_.data(), false);
+        forward(X_, B_, Hs_.data(), G_.data(), Ht_.data(), false);
         const float inv_count = 1.0f / (static_cast<float>(B_) * cfg_.outputs);
         hipLaunchKernelGGL(mlp::loss_grad, dim3(B_ / 64, out_pad_ / 64, H_), dim3(256), 0, stream_, Yp_,
                            Yt_, lossw_, B_, out_pad_, inv_count, dY_, dYt_, lpart_);
@@ -1092,7 +1158,7 @@ class MlpModel {
         // output layer
         {
             const auto &mo = mats_[D];
-            weight_grad(mo, dYt_, Ht_[D - 1], BH);
+            weight_grad(mo, dYt_, Ht_[D - 1], BH, !update);
             hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_ / 4, H_), dim3(256), 0, stream_, dYt_, B_,
                                out_pad_, g_ + mo.boff, P_);
             MLP_TRY(hipGetLastError());
@@ -1106,8 +1172,7 @@ class MlpModel {
             d.M = B_;
             d.N = hid_;
             d.K = out_pad_;
-            d.Zin = Z_[D - 1];
-            d.sZin = BH;
+            d.Gin = G_[D - 1];
             d.Cb = dZ_[0];
             d.Ct = dZt_[0];
             d.sCb = BH;
@@ -1117,7 +1182,7 @@ class MlpModel {
         for (int l = D - 1; l >= 0; --l) {
             const auto &mi = mats_[l];
             weight_grad(mi, dZt_[cur], l == 0 ? Xt_ : Ht_[l - 1],
-                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
+                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH, !update);
             hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_ / 4, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
                                g_ + mi.boff, P_);
             MLP_TRY(hipGetLastError());
@@ -1132,8 +1197,7 @@ class MlpModel {
                 d.M = B_;
                 d.N = hid_;
                 d.K = hid_;
-                d.Zin = Z_[l - 1];
-                d.sZin = BH;
+                d.Gin = G_[l - 1];
                 d.Cb = dZ_[1 - cur];
                 d.Ct = dZt_[1 - cur];
                 d.sCb = BH;
@@ -1146,6 +1210,8 @@ class MlpModel {
             std::memset(&a, 0, sizeof a);
             a.p = p_;
             a.g = g_;
+            a.part = part_;
+            a.splits = splits_;
             a.m = m_;
             a.v = v_;
             a.wb = wb_;
@@ -1161,10 +1227,8 @@ class MlpModel {
             a.step = step_;
             int tiles = 0;
             for (const auto &mi : mats_) tiles += (mi.out / mlp::kAdamTile) * (mi.in / mlp::kAdamTile);
-            hipLaunchKernelGGL(mlp::adam_matrices, dim3(tiles, H_), dim3(256), 0, stream_, a);
-            MLP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(mlp::adam_biases, dim3(static_cast<unsigned>(mats_.size()), H_),
-                               dim3(256), 0, stream_, a);
+            hipLaunchKernelGGL(mlp::adam_params, dim3(tiles + static_cast<int>(mats_.size()), H_),
+                               dim3(256), 0, stream_, a, tiles);
             MLP_TRY(hipGetLastError());
             hipLaunchKernelGGL(mlp::bump_step, dim3(1), dim3(1), 0, stream_, step_);
             MLP_TRY(hipGetLastError());
@@ -1187,7 +1251,7 @@ class MlpModel {
     float *lr_ = nullptr;
     bf16 *X_ = nullptr, *Xt_ = nullptr, *dY_ = nullptr, *dYt_ = nullptr;
     float *Yt_ = nullptr, *Yp_ = nullptr;
-    std::vector<float *> Z_;
+    std::vector<bf16 *> G_;  // GELU'(Z) of each hidden layer (bf16, fragment order)
     std::vector<bf16 *> Hs_, Ht_;
     bf16 *dZ_[2] = {nullptr, nullptr}, *dZt_[2] = {nullptr, nullptr};
     bf16 *Xq_ = nullptr, *Hq_[2] = {nullptr, nullptr};
