@@ -176,17 +176,56 @@ def bench_kernels(queries: int = 1_050_000, train_steps: int = 200, seed: int = 
             "train_mfma_frac": per_step * train_steps / tsec / 2.5e15}
 
 
+def bench_library(rows=(4096, 16384), reps: int = 50) -> dict:
+    """A yardstick for the hidden-layer product: the vendor library (hipBLASLt, through torch's
+    bf16 bmm) on the same shape, 4 heads x [rows x 512] . [512 x 512], alone and followed by the
+    bias add and tanh-GELU as separate operations (what an unfused framework runs). Device times
+    from torch's events on its current stream, microseconds per call."""
+    import torch
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    out = {}
+    for m in rows:
+        a = torch.randn(4, m, 512, device="cuda", dtype=torch.bfloat16, generator=g)
+        w = torch.randn(4, 512, 512, device="cuda", dtype=torch.bfloat16, generator=g)
+        bias = torch.randn(4, 1, 512, device="cuda", dtype=torch.bfloat16, generator=g)
+        c = torch.empty(4, m, 512, device="cuda", dtype=torch.bfloat16)
+        flops = 2 * 4 * m * 512 * 512
+        t_mm = timed(lambda: torch.bmm(a, w, out=c))
+        t_gelu = timed(lambda: torch.nn.functional.gelu(torch.baddbmm(bias, a, w), approximate="tanh"))
+        out[f"rows_{m}"] = {"bmm_us": t_mm, "bmm_tflops": flops / t_mm / 1e6,
+                            "bmm_bias_gelu_us": t_gelu, "bmm_bias_gelu_tflops": flops / t_gelu / 1e6}
+    return out
+
+
 def main(argv=None) -> int:
     import argparse
     import json
 
     ap = argparse.ArgumentParser(prog="python -m tissuemc_amd.mlp",
                                  description="time the surrogate network's kernels")
-    ap.add_argument("command", choices=["bench"])
+    ap.add_argument("command", choices=["bench", "library"],
+                    help="bench: this package's kernels; library: the vendor GEMM on the "
+                         "hidden-layer shape, as a yardstick")
     ap.add_argument("--queries", type=int, default=1_050_000)
     ap.add_argument("--train-steps", type=int, default=200)
     args = ap.parse_args(argv)
-    print(json.dumps(bench_kernels(args.queries, args.train_steps)))
+    if args.command == "library":
+        print(json.dumps(bench_library()))
+    else:
+        print(json.dumps(bench_kernels(args.queries, args.train_steps)))
     return 0
 
 
