@@ -41,6 +41,23 @@ def test_identities_and_no_side_effects(gpu, name, arz):
     assert t["events"] == plain["events"]
 
 
+def test_many_bins_share_lds_with_the_wave_slots(gpu):
+    """The angle and time bins live in each workgroup's LDS ahead of the wave slots: thousands of
+    them take LDS from the A(r,z) tile and, past that, waves from the workgroup; the identities
+    hold all the same. Bins that cannot fit beside even one wave are rejected."""
+    spec = dict(load_spec("v3_three_layer"), na=500)
+    kw = dict(photons=200_000, batches=4, seed=12, arz=True, angles=True)
+    r = engine.run(spec, time=(7000, 0.0002), **kw)  # 2 (500 + 7000) counters = 120 KB
+    fx = r["fixed"]
+    assert int(fx["rt"].sum()) == int(fx["rd"].sum()) == int(fx["ra"].sum())
+    assert int(fx["tt_t"].sum()) == int(fx["tt"].sum()) == int(fx["ta"].sum())
+    plain = engine.run(spec, **kw)
+    for key in ("rd", "tt", "absorbed", "az", "rr", "tr", "arz", "ra", "ta", "rra", "tra"):
+        assert np.array_equal(fx[key], plain["fixed"][key]), key
+    with pytest.raises(engine.ConfigError, match="do not fit in LDS"):
+        engine.run(spec, photons=1000, batches=2, time=(20000, 0.01))
+
+
 def test_non_scattering_slab(gpu):
     n, d, N = 1.5, 0.1, 400_000
     R = ((n - 1.0) / (n + 1.0)) ** 2

@@ -541,18 +541,14 @@ struct AdamArgs {
     const uint64_t *step;
 };
 
-// Adam (Kingma & Ba 2015) with bias correction for one element of gradient g; returns the new
-// parameter.
-__device__ __forceinline__ float adam_step(const AdamArgs &a, float *p, float g, float *m,
-                                           float *v, long long e, float c1, float c2) {
-    const float gr = g + a.wd * p[e];
-    const float mn = a.beta1 * m[e] + (1.0f - a.beta1) * gr;
-    const float vn = a.beta2 * v[e] + (1.0f - a.beta2) * gr * gr;
-    m[e] = mn;
-    v[e] = vn;
-    const float pn = p[e] - a.lr[0] * (mn * c1) / (__builtin_sqrtf(vn * c2) + a.eps);
-    p[e] = pn;
-    return pn;
+// Adam (Kingma & Ba 2015) with bias correction for one element: parameter p, gradient g,
+// moments m and v in, their new values out.
+__device__ __forceinline__ void adam_update(const AdamArgs &a, float &p, float g, float &m,
+                                            float &v, float c1, float c2) {
+    const float gr = g + a.wd * p;
+    m = a.beta1 * m + (1.0f - a.beta1) * gr;
+    v = a.beta2 * v + (1.0f - a.beta2) * gr * gr;
+    p = p - a.lr[0] * (m * c1) / (__builtin_sqrtf(v * c2) + a.eps);
 }
 
 // Adam on every parameter of one head (blockIdx.y). Blocks [0, tiles) take one 64 x 64 tile of
@@ -574,8 +570,14 @@ __global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) 
     if (t >= tiles) {
         const MatInfo mi = a.mats[t - tiles];
         const float *g = a.g + h * a.P;
-        for (int o = tid; o < mi.out; o += static_cast<int>(blockDim.x))
-            adam_step(a, p, g[mi.boff + o], m, v, mi.boff + o, c1, c2);
+        for (int o = tid; o < mi.out; o += static_cast<int>(blockDim.x)) {
+            const long long e = mi.boff + o;
+            float pe = p[e], me = m[e], ve = v[e];
+            adam_update(a, pe, g[e], me, ve, c1, c2);
+            p[e] = pe;
+            m[e] = me;
+            v[e] = ve;
+        }
         return;
     }
     int k = 0;
@@ -589,29 +591,49 @@ __global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) 
     const int o0 = (t / tiles_i) * kAdamTile, i0 = (t % tiles_i) * kAdamTile;
     const float *part = a.part + static_cast<long long>(h) * a.splits * a.P;
     bf16 *wb = a.wb + h * a.Pb;
-    const int c = tid & 63;
-    // rows o0 + r, column i0 + c; four rows at a time, so that their partial loads are in flight
-    // together (at most kMaxSplits partials)
-    for (int r0 = tid >> 6; r0 < kAdamTile; r0 += 16) {
-        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // rows o0 + r, columns i0 + cg .. cg + 3 as 16-byte accesses (every offset is a multiple of 4
+    // floats): thread -> column group tid % 16, rows tid / 16 + 16 q; the partial loads of its
+    // four rows (at most kMaxSplits each) are in flight together
+    const int cg = (tid & 15) * 4, rq = tid >> 4;
+    floatx4 g[4];
 #pragma unroll
-        for (int sp = 0; sp < kMaxSplits; ++sp) {
-            if (sp < a.splits) {
+    for (int q = 0; q < 4; ++q) g[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    g[q] += part[sp * a.P + mi.off + static_cast<long long>(o0 + r0 + 4 * q) * mi.in + i0 + c];
-            }
-        }
+    for (int sp = 0; sp < kMaxSplits; ++sp) {
+        if (sp < a.splits) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = r0 + 4 * q;
-            const long long rel = static_cast<long long>(o0 + r) * mi.in + i0 + c;
-            const bf16 b = to_bf16(adam_step(a, p, g[q], m, v, mi.off + rel, c1, c2));
-            wb[mi.wt_off + rel] = b;
-            tile[c][r] = b;
+            for (int q = 0; q < 4; ++q)
+                g[q] += *reinterpret_cast<const floatx4 *>(
+                    part + sp * a.P + mi.off + static_cast<long long>(o0 + rq + 16 * q) * mi.in + i0 + cg);
         }
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = rq + 16 * q;
+        const long long rel = static_cast<long long>(o0 + r) * mi.in + i0 + cg;
+        floatx4 pe = *reinterpret_cast<const floatx4 *>(p + mi.off + rel);
+        floatx4 me = *reinterpret_cast<const floatx4 *>(m + mi.off + rel);
+        floatx4 ve = *reinterpret_cast<const floatx4 *>(v + mi.off + rel);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float pj = pe[j], mj = me[j], vj = ve[j];
+            adam_update(a, pj, g[q][j], mj, vj, c1, c2);
+            pe[j] = pj;
+            me[j] = mj;
+            ve[j] = vj;
+        }
+        *reinterpret_cast<floatx4 *>(p + mi.off + rel) = pe;
+        *reinterpret_cast<floatx4 *>(m + mi.off + rel) = me;
+        *reinterpret_cast<floatx4 *>(v + mi.off + rel) = ve;
+        const uint32_t b01 = pack_bf16x2(pe[0], pe[1]), b23 = pack_bf16x2(pe[2], pe[3]);
+        *reinterpret_cast<uint2 *>(wb + mi.wt_off + rel) = uint2{b01, b23};
+        tile[cg][r] = static_cast<bf16>(b01);
+        tile[cg + 1][r] = static_cast<bf16>(b01 >> 16);
+        tile[cg + 2][r] = static_cast<bf16>(b23);
+        tile[cg + 3][r] = static_cast<bf16>(b23 >> 16);
+    }
     __syncthreads();
+    const int c = tid & 63;
     for (int r = tid >> 6; r < kAdamTile; r += 4)  // transposed: row i0 + r, column o0 + c
         wb[mi.w_off + static_cast<long long>(i0 + r) * mi.out + o0 + c] = tile[r][c];
 }
