@@ -12,13 +12,14 @@
 // epilogue writes dZ and dZt, and Adam keeps bf16 weights in both orientations (Wt and W).
 // Heads are independent networks batched along gridDim.z. Matrices are padded to multiples of
 // 128 with zeros that stay zero (zero inputs, zero loss weights). Weight gradients reduce over the
-// batch (K = batch), so they are split along K across workgroups and the partials summed in a
-// fixed order by a second kernel: training is bit-reproducible.
+// batch (K = batch), so they are split along K across workgroups; bias gradients are column sums
+// of dZ, made 64 rows at a time in the epilogue that writes dZ. Adam sums both kinds of partials
+// in a fixed order: training is bit-reproducible.
 //
 // MFMA: v_mfma_f32_16x16x32_bf16. Lane l holds A row l%16 and B column l%16, k = 8*(l/16)..+7;
 // the 16x16 result has column l%16, rows 4*(l/16)..+3 (checked on the device). A workgroup of
 // four waves computes a 128x128 tile, each wave 64x64 (4x4 MFMA tiles), K in steps of 32, operands
-// staged through double-buffered LDS tiles (see gemm_nt).
+// staged through a ring of LDS stages (see gemm_nt).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -102,6 +103,9 @@ struct GemmArgs {
     long long sCb;  // head stride of Cb and Ct (M * N)
     bf16 *Gd;         // kFwdHidden (training): GELU'(Z) as bf16, M x N in fragment order
     const bf16 *Gin;  // kBwdData: the GELU'(Z) of this layer, as kFwdHidden wrote it
+    float *Bp;        // kBwdData: the bias gradient's partials, the column sums of dZ over each
+    long long sBp;    // 64 rows: [M / 64 x ldBp] per head (head stride sBp), columns from 0
+    int ldBp;
 };
 
 // Fragment order: the layout in which the training forward hands GELU'(Z) to the backward data
@@ -327,6 +331,14 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
             }
         }
         __syncthreads();
+        if (kEpi == kBwdData && g.Bp) {
+            // the bias gradient's partials: column sums of the bf16 dZ over each 64 rows of the
+            // tile, in row order (thread -> column tid % 128, half tid / 128)
+            const int col = tid & 127, half = tid >> 7;
+            float cs = 0.0f;
+            for (int rr = 0; rr < 64; ++rr) cs += from_bf16(T[(half * 64 + rr) * kPitch + col]);
+            g.Bp[h * g.sBp + static_cast<long long>((mblk >> 6) + half) * g.ldBp + nblk + col] = cs;
+        }
         // 128 rows x 16 pieces of 8 bf16, eight pieces per thread
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -357,21 +369,6 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     }
 }
 
-// Sum of the split-K partials of a weight gradient: g[h][e] = sum_s part[h][s][e] for the
-// elements [off, off + count) of each head's block (fixed order: deterministic).
-__global__ void reduce_splits(const float *part, int splits, long long count, long long s_part_head,
-                              long long s_split, float *g, long long s_g_head) {
-    // four consecutive elements per thread (count, the strides and the offsets are multiples of
-    // 4 floats, so every access is a 16-byte load or store)
-    const int h = static_cast<int>(blockIdx.y);
-    for (long long e = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; e < count;
-         e += static_cast<long long>(gridDim.x) * blockDim.x * 4) {
-        floatx4 s = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int k = 0; k < splits; ++k)
-            s += *reinterpret_cast<const floatx4 *>(part + h * s_part_head + k * s_split + e);
-        *reinterpret_cast<floatx4 *>(g + h * s_g_head + e) = s;
-    }
-}
 
 // Batch assembly for a block of 16 rows of head h: row m takes dataset sample idx (Philox of
 // (step, m, h) mod n); writes the standardized features as bf16 X [B x in_pad] and, through an
@@ -437,13 +434,16 @@ __global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd
 }
 
 // Weighted squared error over the real outputs on a 64-row x 64-column tile of head h: writes dY
-// (bf16) and, through an LDS tile, its transpose dYt (both coalesced), and the tile's share of
-// the loss into part[h][tile] (summed in a fixed order by loss_reduce).
+// (bf16) and, through an LDS tile, its transpose dYt (both coalesced), the tile's share of the
+// loss into part[h][tile] (summed in a fixed order by loss_reduce), and the output bias
+// gradient's partials, the column sums of the bf16 dY over the tile's rows in row order, into
+// bp[h][row block][column] (row stride ld_bp, head stride s_bp).
 __global__ __launch_bounds__(256) void loss_grad(const float *__restrict__ Yp,
                                                  const float *__restrict__ Yt,
                                                  const float *__restrict__ w, int B, int out_pad,
                                                  float inv_count, bf16 *__restrict__ dY,
-                                                 bf16 *__restrict__ dYt, float *__restrict__ part) {
+                                                 bf16 *__restrict__ dYt, float *__restrict__ part,
+                                                 float *__restrict__ bp, long long s_bp, int ld_bp) {
     __shared__ bf16 tile[64][66];
     __shared__ float red[256];
     const int tx = static_cast<int>(threadIdx.x & 63u), ty = static_cast<int>(threadIdx.x >> 6);
@@ -472,6 +472,11 @@ __global__ __launch_bounds__(256) void loss_grad(const float *__restrict__ Yp,
     __syncthreads();
     for (int i = ty; i < 64; i += 4)
         dYt[(static_cast<long long>(h) * out_pad + j0 + i) * B + m0 + tx] = tile[i][tx];
+    if (ty == 0) {
+        float cs = 0.0f;
+        for (int i = 0; i < 64; ++i) cs += from_bf16(tile[tx][i]);
+        bp[h * s_bp + static_cast<long long>(blockIdx.x) * ld_bp + j0 + tx] = cs;
+    }
     for (int off = 128; off > 0; off >>= 1) {
         if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
@@ -498,40 +503,23 @@ __global__ void loss_reduce(const float *part, int n, float inv_count, float *lo
     if (threadIdx.x == 0) loss[static_cast<long long>(step[0] % ring) * heads + h] = red[0] * inv_count;
 }
 
-// Bias gradient: db[h][n] = sum_m dZt[h][n][m] (rows of the transposed gradient are contiguous).
-// One wave per row n, four rows per block: 16-byte loads of 8 bf16, then a butterfly of
-// shuffles in a fixed order (deterministic). B is a multiple of 8 and N of 4.
-__global__ __launch_bounds__(256) void bias_grad(const bf16 *dZt, int B, int N, float *db, long long s_db) {
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    const int n = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
-    const int h = static_cast<int>(blockIdx.y);
-    const uint4 *row = reinterpret_cast<const uint4 *>(dZt + (static_cast<long long>(h) * N + n) * B);
-    float s = 0.0f;
-    for (int q = lane; q < B / 8; q += 64) {
-        const uint4 w = row[q];
-        const uint32_t words[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            s += from_bf16(static_cast<bf16>(words[k])) + from_bf16(static_cast<bf16>(words[k] >> 16));
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) db[h * s_db + n] = s;
-}
-
 struct MatInfo {
     long long off;  // offset of the matrix in a head's parameter block (floats)
     int out, in;    // padded shape [out x in]
     long long boff;  // offset of the bias
     long long wt_off, w_off;  // offsets of the bf16 copies in a head's bf16 block
+    int bb;                   // first column of the bias in the bias-gradient partials
 };
 constexpr int kMaxMats = 8;
 constexpr int kMaxSplits = 8;  // split-K pieces of a weight gradient
 
 struct AdamArgs {
-    float *p, *g, *m, *v;  // [heads x P]; g holds the bias gradients
+    float *p, *g, *m, *v;  // [heads x P]
     const float *part;     // [heads x splits x P]: split-K partials of the matrix gradients
     int splits;
+    const float *bpart;    // [heads x bp_rows x nb]: the bias gradients' partials (64 rows each)
+    int bp_rows, nb;
+    int grad_only;         // write the summed gradients to g instead of updating
     bf16 *wb;              // [heads x Pb] bf16 copies (Wt then W per matrix)
     long long P, Pb;
     int n_mats;
@@ -551,14 +539,17 @@ __device__ __forceinline__ void adam_update(const AdamArgs &a, float &p, float g
     p = p - a.lr[0] * (m * c1) / (__builtin_sqrtf(v * c2) + a.eps);
 }
 
-// Adam on every parameter of one head (blockIdx.y). Blocks [0, tiles) take one 64 x 64 tile of
-// one weight matrix each (the tiles of all matrices in order): the gradient is the sum of the
-// matrix's split-K partials, added in split order as reduce_splits does (so it is the same
-// float), and the block writes the bf16 copies the products read: W [out x in] row by row like
-// the fp32 parameters, and its transpose [in x out] through an LDS tile, so that every global
-// access is coalesced. Block tiles + k updates the bias of matrix k from the gradient block.
+// Adam on every parameter of one head (blockIdx.y). Block k < n_mats updates the bias of matrix
+// k, its gradient the sum of its partials in row-block order (these blocks go first: each sums
+// dozens of partials per element). Blocks n_mats + t take tile t of the weight matrices (64 x 64,
+// the tiles of all matrices in order): the gradient is the sum of the matrix's split-K partials
+// in split order, and the block writes the bf16 copies the products read: W [out x in] row by
+// row like the fp32 parameters, and its transpose [in x out] through an LDS tile, so that every
+// global access is coalesced. With grad_only the blocks write
+// the summed gradients to g and update nothing: the same sums, for a caller that asks for the
+// gradient.
 constexpr int kAdamTile = 64;
-__global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) {
+__global__ __launch_bounds__(256) void adam_params(const AdamArgs a) {
     __shared__ bf16 tile[kAdamTile][kAdamTile + 2];  // [i][o]
     const int h = static_cast<int>(blockIdx.y);
     const float tt = static_cast<float>(a.step[0] + 1);
@@ -567,19 +558,35 @@ __global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) 
     float *p = a.p + h * a.P, *m = a.m + h * a.P, *v = a.v + h * a.P;
     const int tid = static_cast<int>(threadIdx.x);
     int t = static_cast<int>(blockIdx.x);
-    if (t >= tiles) {
-        const MatInfo mi = a.mats[t - tiles];
-        const float *g = a.g + h * a.P;
+    if (t < a.n_mats) {
+        const MatInfo mi = a.mats[t];
+        const float *bp = a.bpart + static_cast<long long>(h) * a.bp_rows * a.nb + mi.bb;
         for (int o = tid; o < mi.out; o += static_cast<int>(blockDim.x)) {
             const long long e = mi.boff + o;
+            // sixteen partial loads in flight at a time, added in row-block order
+            float ge = 0.0f;
+            int rb = 0;
+            for (; rb + 16 <= a.bp_rows; rb += 16) {
+                float pv[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) pv[u] = bp[static_cast<long long>(rb + u) * a.nb + o];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) ge += pv[u];
+            }
+            for (; rb < a.bp_rows; ++rb) ge += bp[static_cast<long long>(rb) * a.nb + o];
+            if (a.grad_only) {
+                a.g[h * a.P + e] = ge;
+                continue;
+            }
             float pe = p[e], me = m[e], ve = v[e];
-            adam_update(a, pe, g[e], me, ve, c1, c2);
+            adam_update(a, pe, ge, me, ve, c1, c2);
             p[e] = pe;
             m[e] = me;
             v[e] = ve;
         }
         return;
     }
+    t -= a.n_mats;
     int k = 0;
     for (; k < a.n_mats; ++k) {
         const int n = (a.mats[k].out / kAdamTile) * (a.mats[k].in / kAdamTile);
@@ -606,6 +613,13 @@ __global__ __launch_bounds__(256) void adam_params(const AdamArgs a, int tiles) 
                 g[q] += *reinterpret_cast<const floatx4 *>(
                     part + sp * a.P + mi.off + static_cast<long long>(o0 + rq + 16 * q) * mi.in + i0 + cg);
         }
+    }
+    if (a.grad_only) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<floatx4 *>(a.g + h * a.P + mi.off +
+                                         static_cast<long long>(o0 + rq + 16 * q) * mi.in + i0 + cg) = g[q];
+        return;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -762,8 +776,10 @@ class MlpModel {
         Bq_ = std::max(B_, 16384);
         // parameter layout per head: W [out x in] then b [out], per matrix
         long long off = 0, boff = 0;
+        nb_ = 0;
         for (int k = 0; k <= c.depth; ++k) {
             mlp::MatInfo mi;
+            mi.bb = nb_;
             mi.in = k == 0 ? in_pad_ : hid_;
             mi.out = k == c.depth ? out_pad_ : hid_;
             mi.off = off;
@@ -774,6 +790,7 @@ class MlpModel {
             boff += static_cast<long long>(mi.out) * mi.in;
             mi.w_off = boff;
             boff += static_cast<long long>(mi.out) * mi.in;
+            nb_ += mi.out;
             mats_.push_back(mi);
             real_in_.push_back(k == 0 ? c.inputs : hid_);
             real_out_.push_back(k == c.depth ? c.outputs : hid_);
@@ -792,6 +809,7 @@ class MlpModel {
         splits_ = std::clamp(B_ / 1024, 1, mlp::kMaxSplits);
         while (B_ % (mlp::kBK * splits_) != 0) --splits_;
         part_ = alloc<float>(static_cast<long long>(H_) * splits_ * P_);
+        bpart_ = alloc<float>(static_cast<long long>(H_) * (B_ / 64) * nb_);
         m_ = alloc<float>(H_ * P_);
         v_ = alloc<float>(H_ * P_);
         wb_ = alloc<bf16>(H_ * Pb_);
@@ -1073,10 +1091,8 @@ class MlpModel {
     }
 
     // dW [out x in] of matrix mi = A (dZt [out x B]) . B (Ht or Xt [in x B])^T, split along the
-    // batch; with `reduce` the partials are summed into the gradient block (a training step
-    // leaves them to Adam, which sums them itself).
-    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht,
-                     bool reduce) {
+    // batch into partials that adam_params sums.
+    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht) {
         mlp::GemmArgs g = base_args();
         g.A = dzt;
         g.lda = B_;
@@ -1092,12 +1108,6 @@ class MlpModel {
         g.sCf = static_cast<long long>(splits_) * P_;
         g.sSplit = P_;
         gemm<mlp::kBwdWeight>(g);
-        if (!reduce) return;
-        const long long count = static_cast<long long>(mi.out) * mi.in;
-        hipLaunchKernelGGL(mlp::reduce_splits, dim3(static_cast<unsigned>(std::min<long long>((count / 4 + 255) / 256, 1024)), H_),
-                           dim3(256), 0, stream_, part_ + mi.off, splits_, count,
-                           static_cast<long long>(splits_) * P_, P_, g_ + mi.off, P_);
-        MLP_TRY(hipGetLastError());
     }
 
     mlp::GemmArgs base_args() const {
@@ -1169,7 +1179,8 @@ class MlpModel {
         forward(X_, B_, Hs_.data(), G_.data(), Ht_.data(), false);
         const float inv_count = 1.0f / (static_cast<float>(B_) * cfg_.outputs);
         hipLaunchKernelGGL(mlp::loss_grad, dim3(B_ / 64, out_pad_ / 64, H_), dim3(256), 0, stream_, Yp_,
-                           Yt_, lossw_, B_, out_pad_, inv_count, dY_, dYt_, lpart_);
+                           Yt_, lossw_, B_, out_pad_, inv_count, dY_, dYt_, lpart_,
+                           bpart_ + mats_[cfg_.depth].bb, static_cast<long long>(B_ / 64) * nb_, nb_);
         MLP_TRY(hipGetLastError());
         hipLaunchKernelGGL(mlp::loss_reduce, dim3(H_), dim3(256), 0, stream_, lpart_,
                            (B_ / 64) * (out_pad_ / 64), inv_count, loss_, step_, H_,
@@ -1180,10 +1191,7 @@ class MlpModel {
         // output layer
         {
             const auto &mo = mats_[D];
-            weight_grad(mo, dYt_, Ht_[D - 1], BH, !update);
-            hipLaunchKernelGGL(mlp::bias_grad, dim3(out_pad_ / 4, H_), dim3(256), 0, stream_, dYt_, B_,
-                               out_pad_, g_ + mo.boff, P_);
-            MLP_TRY(hipGetLastError());
+            weight_grad(mo, dYt_, Ht_[D - 1], BH);
             mlp::GemmArgs d = base_args();
             d.A = dY_;
             d.lda = out_pad_;
@@ -1195,6 +1203,7 @@ class MlpModel {
             d.N = hid_;
             d.K = out_pad_;
             d.Gin = G_[D - 1];
+            set_bias_partials(d, mats_[D - 1]);
             d.Cb = dZ_[0];
             d.Ct = dZt_[0];
             d.sCb = BH;
@@ -1204,10 +1213,7 @@ class MlpModel {
         for (int l = D - 1; l >= 0; --l) {
             const auto &mi = mats_[l];
             weight_grad(mi, dZt_[cur], l == 0 ? Xt_ : Ht_[l - 1],
-                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH, !update);
-            hipLaunchKernelGGL(mlp::bias_grad, dim3(hid_ / 4, H_), dim3(256), 0, stream_, dZt_[cur], B_, hid_,
-                               g_ + mi.boff, P_);
-            MLP_TRY(hipGetLastError());
+                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
             if (l > 0) {
                 mlp::GemmArgs d = base_args();
                 d.A = dZ_[cur];
@@ -1220,6 +1226,7 @@ class MlpModel {
                 d.N = hid_;
                 d.K = hid_;
                 d.Gin = G_[l - 1];
+                set_bias_partials(d, mats_[l - 1]);
                 d.Cb = dZ_[1 - cur];
                 d.Ct = dZt_[1 - cur];
                 d.sCb = BH;
@@ -1227,13 +1234,18 @@ class MlpModel {
                 cur = 1 - cur;
             }
         }
-        if (update) {
+        // Adam, or (without `update`) the summed gradients into g_
+        {
             mlp::AdamArgs a;
             std::memset(&a, 0, sizeof a);
             a.p = p_;
             a.g = g_;
             a.part = part_;
             a.splits = splits_;
+            a.bpart = bpart_;
+            a.bp_rows = B_ / 64;
+            a.nb = nb_;
+            a.grad_only = update ? 0 : 1;
             a.m = m_;
             a.v = v_;
             a.wb = wb_;
@@ -1250,11 +1262,21 @@ class MlpModel {
             int tiles = 0;
             for (const auto &mi : mats_) tiles += (mi.out / mlp::kAdamTile) * (mi.in / mlp::kAdamTile);
             hipLaunchKernelGGL(mlp::adam_params, dim3(tiles + static_cast<int>(mats_.size()), H_),
-                               dim3(256), 0, stream_, a, tiles);
+                               dim3(256), 0, stream_, a);
             MLP_TRY(hipGetLastError());
+        }
+        if (update) {
             hipLaunchKernelGGL(mlp::bump_step, dim3(1), dim3(1), 0, stream_, step_);
             MLP_TRY(hipGetLastError());
         }
+    }
+
+    // The bias-gradient partials of layer mi, written by the backward data product that makes
+    // its dZ.
+    void set_bias_partials(mlp::GemmArgs &d, const mlp::MatInfo &mi) const {
+        d.Bp = bpart_ + mi.bb;
+        d.sBp = static_cast<long long>(B_ / 64) * nb_;
+        d.ldBp = nb_;
     }
 
     tmc_mlp_config cfg_;
@@ -1265,6 +1287,8 @@ class MlpModel {
     std::vector<int> real_in_, real_out_;
     std::vector<void *> allocs_;
     int splits_ = 1;
+    float *bpart_ = nullptr;  // bias-gradient partials [H x B / 64 x nb_]
+    int nb_ = 0;              // biases per head
     float *part_ = nullptr;
     float *p_ = nullptr, *g_ = nullptr, *m_ = nullptr, *v_ = nullptr, *lossw_ = nullptr, *loss_ = nullptr;
     float *lpart_ = nullptr;  // per-tile loss partials [heads x tiles]
