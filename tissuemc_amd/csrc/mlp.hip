@@ -2,14 +2,14 @@
 // kernels (FP32 accumulation) on gfx950. No reference counterpart.
 //
 // Network (per ensemble head): inputs -> depth x [hidden, GELU] -> outputs, GELU in its tanh
-// form. Every matrix product is one kernel, `gemm_nt`: C[m, n] = sum_k A[m, k] * B[n, k] with
-// both operands K-contiguous in memory (the layout the MFMA wants per lane), so the three
-// products of a layer read these layouts:
-//   forward          Z = H_prev . W^T     A = H_prev [batch x in],  B = Wt [out x in]
-//   input gradient   dH = dZ . W          A = dZ [batch x out],     B = W  [in x out]
-//   weight gradient  dW = dZ^T . H_prev   A = dZt [out x batch],    B = Ht_prev [in x batch]
-// so the forward epilogue writes activations in both orientations (H and Ht), the backward
-// epilogue writes dZ and dZt, and Adam keeps bf16 weights in both orientations (Wt and W).
+// form. The forward and input-gradient products are `gemm_nt`: C[m, n] = sum_k A[m, k] * B[n, k]
+// with both operands K-contiguous in memory (the layout the MFMA wants per lane); the weight
+// gradient, whose K is the batch, is `gemm_tn`: C[m, n] = sum_k A[k, m] * B[k, n], reading the
+// activations and gradients as the other two wrote them and transposing in LDS:
+//   forward          Z = H_prev . W^T     gemm_nt  A = H_prev [batch x in],  B = Wt [out x in]
+//   input gradient   dH = dZ . W          gemm_nt  A = dZ [batch x out],     B = W  [in x out]
+//   weight gradient  dW = dZ^T . H_prev   gemm_tn  A = dZ [batch x out],     B = H_prev [batch x in]
+// so Adam keeps bf16 weights in both orientations (Wt and W) and nothing else is transposed.
 // Heads are independent networks batched along gridDim.z. Matrices are padded to multiples of
 // 128 with zeros that stay zero (zero inputs, zero loss weights). Weight gradients reduce over the
 // batch (K = batch), so they are split along K across workgroups; bias gradients are column sums
@@ -84,23 +84,22 @@ __device__ __forceinline__ float gelu_grad_s(float z, float s) {
 
 // kInferHidden is kFwdHidden for inference: no Z and no transposed copy, so its results need no
 // LDS staging (below).
-enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kBwdWeight = 3, kInferHidden = 4 };
+enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kInferHidden = 4 };
 
 struct GemmArgs {
     const bf16 *A;
     const bf16 *B;
     int M, N, K, lda, ldb;
     long long sA, sB;  // head strides (elements)
-    int splits;        // split-K factor (kBwdWeight only); blockIdx.z = head * splits + split
+    int splits;        // split-K factor (gemm_tn); blockIdx.z = head * splits + split
     // epilogue operands (head strides in elements)
     const float *bias;
     long long sBias;
-    float *Cf;  // fp32 output [M x N] (Y for kFwdOut, dW partials for kBwdWeight)
+    float *Cf;  // fp32 output [M x N] (Y for kFwdOut, dW partials for gemm_tn)
     long long sCf;
-    long long sSplit;  // kBwdWeight: stride between split partials
+    long long sSplit;  // gemm_tn: stride between split partials
     bf16 *Cb;   // bf16 output [M x N] (H or dZ)
-    bf16 *Ct;   // bf16 transposed output [N x M] (Ht or dZt)
-    long long sCb;  // head stride of Cb and Ct (M * N)
+    long long sCb;  // head stride of Cb (M * N)
     bf16 *Gd;         // kFwdHidden (training): GELU'(Z) as bf16, M x N in fragment order
     const bf16 *Gin;  // kBwdData: the GELU'(Z) of this layer, as kFwdHidden wrote it
     float *Bp;        // kBwdData: the bias gradient's partials, the column sums of dZ over each
@@ -134,22 +133,24 @@ __device__ __forceinline__ long long frag_off(int mblk, int nblk, int N, int wav
 // whether the LDS serves 8 or 16 lanes of a 16-byte read per cycle. The loads pick their global
 // chunk to match, since a lane's destination in LDS is fixed by its lane number.
 //
-// Epilogues that write bf16 H / dZ together with their transposes stage the tile through LDS
-// (68 KB, so two workgroups per CU). The others store straight from registers and use only the
-// ring; for those the MFMA operands are swapped (the tile is computed as B A^T), so that a lane
-// holds four consecutive columns of one row: 16 bytes of fp32, or 8 bytes of bf16, per store.
-// Their ring has three stages (48 KB): three workgroups per CU, which the registers also allow.
+// The training epilogues, which write bf16 H / dZ, stage the tile through LDS (34 KB, inside
+// the 64 KB of their four-stage ring: two workgroups per CU). The others store straight from
+// registers and use only the ring; for those the MFMA operands are swapped (the tile is computed
+// as B A^T), so that a lane holds four consecutive columns of one row: 16 bytes of fp32, or 8
+// bytes of bf16, per store. Their ring has three stages (48 KB): three workgroups per CU, which
+// the registers also allow.
 constexpr int kBK = 32;                          // K per stage
 constexpr int kStageElems = 2 * 128 * kBK;       // bf16 elements of one stage (A then B)
 constexpr int kPitch = 136;                      // bf16 row pitch of the epilogue's output tiles
-constexpr int kGemmLds = 2 * 128 * kPitch * 2;   // bytes: the epilogue tiles of H and H^T
+constexpr int kGemmLds = 128 * kPitch * 2;       // bytes: the epilogue's output tile
 constexpr bool lds_epilogue(int epi) { return epi == kFwdHidden || epi == kBwdData; }
 constexpr bool swapped(int epi) { return epi == kInferHidden || epi == kFwdOut; }
 constexpr int stages_of(int epi) { return lds_epilogue(epi) ? 4 : 3; }
 constexpr int lds_bytes_of(int epi) {
-    return lds_epilogue(epi) ? kGemmLds : stages_of(epi) * kStageElems * 2;
+    return lds_epilogue(epi) && kGemmLds > stages_of(epi) * kStageElems * 2
+               ? kGemmLds
+               : stages_of(epi) * kStageElems * 2;
 }
-static_assert(kGemmLds >= 4 * kStageElems * 2, "epilogue tiles must cover the staging ring");
 
 __device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
@@ -178,8 +179,9 @@ __device__ __forceinline__ void stage_sync_ahead(int ahead) {
 
 template <int kEpi>
 __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
+    static_assert(swapped(kEpi) || lds_epilogue(kEpi), "an epilogue of each kind");
     constexpr int kStages = stages_of(kEpi);
-    extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // staging ring, then T and Tt
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];  // staging ring, then T
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -287,12 +289,10 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     // Epilogue: in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3.
     if constexpr (kEpi == kFwdHidden || kEpi == kBwdData) {
         // bf16 results go through LDS (free after the last stage's barrier): written row-major
-        // into T and, when the transpose is wanted, column-major into Tt, then read back in
-        // 16-byte row pieces, so that every global store is coalesced. The pitch of 136 puts the
-        // four row groups of a fragment 16 banks apart.
-        bf16 *T = smem, *Tt = smem + 128 * kPitch;
+        // into T, then read back in 16-byte row pieces, so that every global store is coalesced.
+        // The pitch of 136 puts the four row groups of a fragment 16 banks apart.
+        bf16 *T = smem;
         bf16 *Cb = g.Cb + h * g.sCb;
-        bf16 *Ct = g.Ct ? g.Ct + h * g.sCb : nullptr;
         const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -327,7 +327,6 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
                 T[(rb + 1) * kPitch + col] = static_cast<bf16>(p01 >> 16);
                 T[(rb + 2) * kPitch + col] = static_cast<bf16>(p23);
                 T[(rb + 3) * kPitch + col] = static_cast<bf16>(p23 >> 16);
-                if (Ct) *reinterpret_cast<uint2 *>(Tt + col * kPitch + rb) = uint2{p01, p23};
             }
         }
         __syncthreads();
@@ -346,36 +345,163 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
             *reinterpret_cast<uint4 *>(Cb + static_cast<long long>(mblk + row) * g.N + nblk + piece) =
                 *reinterpret_cast<const uint4 *>(T + row * kPitch + piece);
         }
-        if (Ct) {
+    }
+}
+
+// The weight gradient without transposed copies: C[m, n] = sum_k A[k, m] B[k, n] with A = dZ
+// [batch x out] and B = H [batch x in] as the forward and backward products wrote them (K = the
+// batch, the row index of both), split along K across gridDim.z, fp32 partials.
+// Tiles and waves as in gemm_nt: 128 x 128 per workgroup, 64 x 64 per wave, K in stages of 32.
+//
+// A stage holds 32 k-rows x 128 columns of each operand, copied row by row from global memory
+// into LDS (global_load_lds_dwordx4: rows of 256 bytes, 16 lanes a row, coalesced); the 16-byte
+// chunk c of row r is stored at chunk c ^ 2 (r & 7). The MFMA wants 8 consecutive k of one column
+// per lane, which ds_read_b64_tr_b16 delivers from this layout: within each 16 lanes, lane i
+// receives element i % 4 of the 8 bytes that lane 4 j + i / 4 addresses, as its element j
+// (j = 0..3). So lane L of group g addresses row 8 g + L / 4 (then 8 g + 4 + L / 4), columns
+// c0 + 4 (L % 4) .. + 3 of the fragment's 16 columns c0 .. c0 + 15, and receives column c0 + L at
+// k = 8 g .. 8 g + 3 (then 8 g + 4 .. 8 g + 7): the fragment the MFMA reads. The swizzle puts the
+// four rows a group reads in each instruction on distinct banks.
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+constexpr int kTnStages = 3;
+constexpr int kTnLds = kTnStages * kStageElems * 2;  // bytes
+__device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 7); }
+
+// Eight ds_read_b64_tr_b16 at LDS byte addresses a[0..7], and the wait for them, in one block.
+// Written as asm because the builtin's reads are not tied to the stage they read, so the compiler
+// waits for every global-to-LDS copy in flight (the next stages' too) before them. The block
+// ends only when the reads have landed; its results are early-clobber, so no address register
+// is reused for a result while reads are still being issued.
+__device__ __forceinline__ void tr_read8(shortx4 (&o)[8], const uint32_t (&a)[8]) {
+    asm volatile(
+        "ds_read_b64_tr_b16 %0, %8\n\t"
+        "ds_read_b64_tr_b16 %1, %9\n\t"
+        "ds_read_b64_tr_b16 %2, %10\n\t"
+        "ds_read_b64_tr_b16 %3, %11\n\t"
+        "ds_read_b64_tr_b16 %4, %12\n\t"
+        "ds_read_b64_tr_b16 %5, %13\n\t"
+        "ds_read_b64_tr_b16 %6, %14\n\t"
+        "ds_read_b64_tr_b16 %7, %15\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]),
+          "=&v"(o[6]), "=&v"(o[7])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7])
+        : "memory");
+}
+
+__global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int z = static_cast<int>(blockIdx.z);
+    const int h = z / g.splits, split = z % g.splits;
+    const int NX = static_cast<int>(gridDim.x), T = NX * static_cast<int>(gridDim.y);
+    int tile = static_cast<int>(blockIdx.x) + NX * static_cast<int>(blockIdx.y);
+    if (T % 8 == 0) tile = (tile % 8) * (T / 8) + tile / 8;  // XCD-aware, as in gemm_nt
+    const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 128;
+    const int kspan = g.K / g.splits;
+    const int kbeg = split * kspan;
+    const int nk = kspan / kBK;
+    const bf16 *A = g.A + h * g.sA;
+    const bf16 *B = g.B + h * g.sB;
+
+    // loads: wave w moves rows 8 w + 4 q + lane / 16 (q = 0, 1) of A and of B; the lane's LDS
+    // slot is chunk lane % 16 of its row, so it fetches the chunk stored there
+    const int lr0 = 8 * wave + (lane >> 4), lr1 = lr0 + 4;
+    const int lc0 = ((lane & 15) ^ tn_swz(lr0)) * 8, lc1 = ((lane & 15) ^ tn_swz(lr1)) * 8;
+    const bf16 *a0 = A + static_cast<long long>(kbeg + lr0) * g.lda + mblk + lc0;
+    const bf16 *a1 = A + static_cast<long long>(kbeg + lr1) * g.lda + mblk + lc1;
+    const bf16 *b0 = B + static_cast<long long>(kbeg + lr0) * g.ldb + nblk + lc0;
+    const bf16 *b1 = B + static_cast<long long>(kbeg + lr1) * g.ldb + nblk + lc1;
+    const int wdst = 8 * wave * 128;  // the wave's first row in a tile (elements)
+
+    // transposed fragment reads: lane L of group gq reads rows 8 gq + L / 4 (+ 4), columns
+    // c0 + 4 (L % 4), c0 = the fragment's first column (a multiple of 16)
+    const int gq = lane >> 4, L = lane & 15;
+    const int fr0 = 8 * gq + (L >> 2), fr1 = fr0 + 4;
+    const int fcol = 4 * (L & 3);  // column offset inside the fragment
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    auto tr_off = [&](int row, int c0) {  // byte offset in a tile of (row, c0 + fcol)
+        const int col = c0 + fcol;
+        return static_cast<uint32_t>(2 * (row * 128 + (((col >> 3) ^ tn_swz(row)) << 3) + (col & 7)));
+    };
+    // reads 0..3: k = 8 gq .. + 3 of fragments 0..3; reads 4..7: k = 8 gq + 4 .. + 7
+    uint32_t aoff[8], boff[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int c = tid + 256 * k, row = c >> 4, piece = (c & 15) * 8;
-                *reinterpret_cast<uint4 *>(Ct + static_cast<long long>(nblk + row) * g.M + mblk + piece) =
-                    *reinterpret_cast<const uint4 *>(Tt + row * kPitch + piece);
-            }
+    for (int i = 0; i < 4; ++i) {
+        aoff[i] = tr_off(fr0, wm + 16 * i);
+        aoff[4 + i] = tr_off(fr1, wm + 16 * i);
+        boff[i] = tr_off(fr0, wn + 16 * i);
+        boff[4 + i] = tr_off(fr1, wn + 16 * i);
+    }
+    const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_void *)smem));
+
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+#define TMC_TN_ISSUE(st)                                                                          \
+    do {                                                                                          \
+        bf16 *As_ = smem + ((st) % kTnStages) * kStageElems + wdst, *Bs_ = As_ + 32 * 128;        \
+        const long long ko = static_cast<long long>(st) * kBK;                                    \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a0 + ko * g.lda), (lds_void *)As_, 16, 0, 0);        \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a1 + ko * g.lda), (lds_void *)(As_ + 4 * 128), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + ko * g.ldb), (lds_void *)Bs_, 16, 0, 0);        \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b1 + ko * g.ldb), (lds_void *)(Bs_ + 4 * 128), 16, 0, 0); \
+    } while (0)
+
+    for (int st = 0; st < kTnStages - 1 && st < nk; ++st) TMC_TN_ISSUE(st);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int ahead = nk - 1 - kt < kTnStages - 2 ? nk - 1 - kt : kTnStages - 2;
+        stage_sync_ahead<kTnStages - 2>(ahead);
+        if (kt + kTnStages - 1 < nk) TMC_TN_ISSUE(kt + kTnStages - 1);
+        const uint32_t abase = lds0 + static_cast<uint32_t>((kt % kTnStages) * kStageElems * 2);
+        const uint32_t bbase = abase + 32 * 128 * 2;
+        uint32_t aa[8], ab[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            aa[u] = abase + aoff[u];
+            ab[u] = bbase + boff[u];
         }
-    } else {
+        shortx4 ra[8], rb[8];
+        tr_read8(ra, aa);
+        tr_read8(rb, ab);
+        bf16x8 fa[4], fb[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
+            fa[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(ra[i], ra[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+            fb[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rb[i], rb[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+        }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = n0 + 16 * j + r;
-                const int mb = m0 + 16 * i + (lane >> 4) * 4;
-                float *C = g.Cf + h * g.sCf + split * g.sSplit;  // kBwdWeight
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
-            }
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+#undef TMC_TN_ISSUE
+    // in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3
+    float *C = g.Cf + h * g.sCf + split * g.sSplit;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = nblk + wn + 16 * j + (lane & 15);
+            const int mb = mblk + wm + 16 * i + (lane >> 4) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
         }
     }
 }
 
 
 // Batch assembly for a block of 16 rows of head h: row m takes dataset sample idx (Philox of
-// (step, m, h) mod n); writes the standardized features as bf16 X [B x in_pad] and, through an
-// LDS tile, Xt [in_pad x B], and the targets padded to out_pad as fp32 Y [B x out_pad].
-// Consecutive threads take consecutive columns, so every access is coalesced. Only the first
-// in_used columns of X (rows of Xt) are written, zero past `in`: the rest stay as allocated
-// (zero). The rows are random, so each read is a separate trip to memory: a thread issues all of
+// (step, m, h) mod n); writes the standardized features as bf16 X [B x in_pad] and the targets
+// padded to out_pad as fp32 Y [B x out_pad]. Consecutive threads take consecutive columns, so
+// every access is coalesced. Only the first in_used columns of X are written, zero past `in`:
+// the rest stay as allocated (zero). The rows are random, so each read is a separate trip to memory: a thread issues all of
 // its reads before its first store, and waits for them once rather than once per row. in_used is
 // a multiple of 32 and at most in_pad.
 constexpr int kGatherRows = 16;
@@ -385,10 +511,9 @@ __global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd
                                                     int out_pad, int B,
                                                     const uint64_t *step, uint32_t key0,
                                                     uint32_t key1, bf16 *__restrict__ X,
-                                                    bf16 *__restrict__ Xt, float *__restrict__ Y,
+                                                    float *__restrict__ Y,
                                                     const int64_t *fixed_rows) {
     __shared__ int64_t rows[kGatherRows];
-    __shared__ bf16 tile[32][kGatherRows + 2];  // [column][row] of one 32-column slab
     const int tid = static_cast<int>(threadIdx.x);
     const int m0 = static_cast<int>(blockIdx.x) * kGatherRows;
     const int h = static_cast<int>(blockIdx.y);
@@ -406,7 +531,6 @@ __global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd
     }
     __syncthreads();
     bf16 *Xh = X + static_cast<long long>(h) * B * in_pad;
-    bf16 *Xth = Xt + static_cast<long long>(h) * B * in_pad;
     float *Yh = Y + static_cast<long long>(h) * B * out_pad;
     for (int j0 = 0; j0 < in_used; j0 += 32) {
         // 16 rows x 32 columns: thread -> column tid % 32, rows tid / 32 and tid / 32 + 8
@@ -416,13 +540,6 @@ __global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd
         const bf16 b0 = to_bf16(v0), b1 = to_bf16(v1);
         Xh[static_cast<long long>(m0 + i0) * in_pad + j] = b0;
         Xh[static_cast<long long>(m0 + i0 + 8) * in_pad + j] = b1;
-        tile[tid & 31][i0] = b0;
-        tile[tid & 31][i0 + 8] = b1;
-        __syncthreads();
-        // Xt rows j0 .. j0 + 31, 16 rows of X each: thread -> (column tid / 16 + 16 k, row tid % 16)
-        for (int c = tid >> 4; c < 32; c += 16)
-            Xth[static_cast<long long>(j0 + c) * B + m0 + (tid & 15)] = tile[c][tid & 15];
-        __syncthreads();
     }
     for (int j = tid; j < out_pad; j += 256) {
         float v[kGatherRows];
@@ -434,15 +551,15 @@ __global__ __launch_bounds__(256) void gather_batch(const float *__restrict__ xd
 }
 
 // Weighted squared error over the real outputs on a 64-row x 64-column tile of head h: writes dY
-// (bf16) and, through an LDS tile, its transpose dYt (both coalesced), the tile's share of the
-// loss into part[h][tile] (summed in a fixed order by loss_reduce), and the output bias
-// gradient's partials, the column sums of the bf16 dY over the tile's rows in row order, into
-// bp[h][row block][column] (row stride ld_bp, head stride s_bp).
+// (bf16, coalesced), the tile's share of the loss into part[h][tile] (summed in a fixed order by
+// loss_reduce), and the output bias gradient's partials, the column sums of the bf16 dY over the
+// tile's rows in row order (through an LDS tile), into bp[h][row block][column] (row stride
+// ld_bp, head stride s_bp).
 __global__ __launch_bounds__(256) void loss_grad(const float *__restrict__ Yp,
                                                  const float *__restrict__ Yt,
                                                  const float *__restrict__ w, int B, int out_pad,
                                                  float inv_count, bf16 *__restrict__ dY,
-                                                 bf16 *__restrict__ dYt, float *__restrict__ part,
+                                                 float *__restrict__ part,
                                                  float *__restrict__ bp, long long s_bp, int ld_bp) {
     __shared__ bf16 tile[64][66];
     __shared__ float red[256];
@@ -470,8 +587,6 @@ __global__ __launch_bounds__(256) void loss_grad(const float *__restrict__ Yp,
     }
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int i = ty; i < 64; i += 4)
-        dYt[(static_cast<long long>(h) * out_pad + j0 + i) * B + m0 + tx] = tile[i][tx];
     if (ty == 0) {
         float cs = 0.0f;
         for (int i = 0; i < 64; ++i) cs += from_bf16(tile[tx][i]);
@@ -765,7 +880,6 @@ class MlpModel {
         set_lds<mlp::kFwdHidden>();
         set_lds<mlp::kFwdOut>();
         set_lds<mlp::kBwdData>();
-        set_lds<mlp::kBwdWeight>();
         set_lds<mlp::kInferHidden>();
         H_ = c.heads;
         in_pad_ = pad128(c.inputs);
@@ -820,19 +934,15 @@ class MlpModel {
         lpart_ = alloc<float>(static_cast<long long>(H_) * (B_ / 64) * (out_pad_ / 64));
         const long long BH = static_cast<long long>(B_) * hid_;
         X_ = alloc<bf16>(H_ * B_ * in_pad_);
-        Xt_ = alloc<bf16>(H_ * B_ * in_pad_);
         Yt_ = alloc<float>(H_ * B_ * out_pad_);
         Yp_ = alloc<float>(H_ * static_cast<long long>(Bq_) * out_pad_);
         dY_ = alloc<bf16>(H_ * B_ * out_pad_);
-        dYt_ = alloc<bf16>(H_ * B_ * out_pad_);
         for (int l = 0; l < c.depth; ++l) {
             G_.push_back(alloc<bf16>(H_ * BH));
             Hs_.push_back(alloc<bf16>(H_ * BH));
-            Ht_.push_back(alloc<bf16>(H_ * BH));
         }
         for (int i = 0; i < 2; ++i) {
             dZ_[i] = alloc<bf16>(H_ * BH);
-            dZt_[i] = alloc<bf16>(H_ * BH);
         }
         const long long QH = static_cast<long long>(Bq_) * hid_;
         Xq_ = alloc<bf16>(H_ * static_cast<long long>(Bq_) * in_pad_);
@@ -954,7 +1064,7 @@ class MlpModel {
             MLP_TRY(hipEventRecord(e0_, stream_));
             hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (in_pad_ / 8) + 255) / 256), dim3(256), 0,
                                stream_, xq_, rows, cfg_.inputs, in_pad_, Bq_, Xq_);
-            forward(Xq_, Bq_, Hq_, nullptr, nullptr, true);
+            forward(Xq_, Bq_, Hq_, nullptr, true);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             MLP_TRY(hipGetLastError());
@@ -1090,16 +1200,18 @@ class MlpModel {
         MLP_TRY(hipGetLastError());
     }
 
-    // dW [out x in] of matrix mi = A (dZt [out x B]) . B (Ht or Xt [in x B])^T, split along the
-    // batch into partials that adam_params sums.
-    void weight_grad(const mlp::MatInfo &mi, const bf16 *dzt, const bf16 *ht, long long s_ht) {
+    // dW [out x in] of matrix mi = dZ^T . H with dZ [B x out] (head stride s_dz) and H, the
+    // layer's input, [B x in] (head stride s_h), split along the batch into partials that
+    // adam_params sums.
+    void weight_grad(const mlp::MatInfo &mi, const bf16 *dz, long long s_dz, const bf16 *h,
+                     long long s_h) {
         mlp::GemmArgs g = base_args();
-        g.A = dzt;
-        g.lda = B_;
-        g.sA = static_cast<long long>(mi.out) * B_;
-        g.B = ht;
-        g.ldb = B_;
-        g.sB = s_ht;
+        g.A = dz;
+        g.lda = mi.out;
+        g.sA = s_dz;
+        g.B = h;
+        g.ldb = mi.in;
+        g.sB = s_h;
         g.M = mi.out;
         g.N = mi.in;
         g.K = B_;
@@ -1107,7 +1219,11 @@ class MlpModel {
         g.Cf = part_ + mi.off;
         g.sCf = static_cast<long long>(splits_) * P_;
         g.sSplit = P_;
-        gemm<mlp::kBwdWeight>(g);
+        if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
+            throw EngineError(TMC_ERR_RUNTIME, "gemm_tn shape is not a multiple of the tile");
+        hipLaunchKernelGGL(mlp::gemm_tn, dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256),
+                           mlp::kTnLds, stream_, g);
+        MLP_TRY(hipGetLastError());
     }
 
     mlp::GemmArgs base_args() const {
@@ -1118,7 +1234,7 @@ class MlpModel {
 
     // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
     // alternate between two buffers and neither Z nor Ht is kept.
-    void forward(bf16 *X, int Bn, bf16 *const *Hbuf, bf16 *const *Gbuf, bf16 *const *Htbuf, bool infer) {
+    void forward(bf16 *X, int Bn, bf16 *const *Hbuf, bf16 *const *Gbuf, bool infer) {
         const long long BH = static_cast<long long>(Bn) * hid_;
         const bf16 *in = X;
         int in_dim = in_pad_;
@@ -1140,7 +1256,6 @@ class MlpModel {
             g.sBias = P_;
             g.Gd = infer ? nullptr : Gbuf[l];
             g.Cb = infer ? Hbuf[l % 2] : Hbuf[l];
-            g.Ct = infer ? nullptr : Htbuf[l];
             g.sCb = BH;
             if (infer)
                 gemm<mlp::kInferHidden>(g);
@@ -1173,13 +1288,13 @@ class MlpModel {
         const uint32_t k1 = static_cast<uint32_t>((cfg_.seed >> 32) ^ 0xB5297A4DULL);
         hipLaunchKernelGGL(mlp::gather_batch, dim3(B_ / mlp::kGatherRows, H_), dim3(256), 0, stream_, xd_,
                            yd_, n_data_,
-                           cfg_.inputs, cfg_.outputs, k0_, in_pad_, out_pad_, B_, step_, k0, k1, X_, Xt_,
+                           cfg_.inputs, cfg_.outputs, k0_, in_pad_, out_pad_, B_, step_, k0, k1, X_,
                            Yt_, fixed_rows);
         MLP_TRY(hipGetLastError());
-        forward(X_, B_, Hs_.data(), G_.data(), Ht_.data(), false);
+        forward(X_, B_, Hs_.data(), G_.data(), false);
         const float inv_count = 1.0f / (static_cast<float>(B_) * cfg_.outputs);
         hipLaunchKernelGGL(mlp::loss_grad, dim3(B_ / 64, out_pad_ / 64, H_), dim3(256), 0, stream_, Yp_,
-                           Yt_, lossw_, B_, out_pad_, inv_count, dY_, dYt_, lpart_,
+                           Yt_, lossw_, B_, out_pad_, inv_count, dY_, lpart_,
                            bpart_ + mats_[cfg_.depth].bb, static_cast<long long>(B_ / 64) * nb_, nb_);
         MLP_TRY(hipGetLastError());
         hipLaunchKernelGGL(mlp::loss_reduce, dim3(H_), dim3(256), 0, stream_, lpart_,
@@ -1191,7 +1306,7 @@ class MlpModel {
         // output layer
         {
             const auto &mo = mats_[D];
-            weight_grad(mo, dYt_, Ht_[D - 1], BH);
+            weight_grad(mo, dY_, static_cast<long long>(B_) * out_pad_, Hs_[D - 1], BH);
             mlp::GemmArgs d = base_args();
             d.A = dY_;
             d.lda = out_pad_;
@@ -1205,14 +1320,13 @@ class MlpModel {
             d.Gin = G_[D - 1];
             set_bias_partials(d, mats_[D - 1]);
             d.Cb = dZ_[0];
-            d.Ct = dZt_[0];
             d.sCb = BH;
             gemm<mlp::kBwdData>(d);
         }
         int cur = 0;
         for (int l = D - 1; l >= 0; --l) {
             const auto &mi = mats_[l];
-            weight_grad(mi, dZt_[cur], l == 0 ? Xt_ : Ht_[l - 1],
+            weight_grad(mi, dZ_[cur], BH, l == 0 ? X_ : Hs_[l - 1],
                         l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
             if (l > 0) {
                 mlp::GemmArgs d = base_args();
@@ -1228,7 +1342,6 @@ class MlpModel {
                 d.Gin = G_[l - 1];
                 set_bias_partials(d, mats_[l - 1]);
                 d.Cb = dZ_[1 - cur];
-                d.Ct = dZt_[1 - cur];
                 d.sCb = BH;
                 gemm<mlp::kBwdData>(d);
                 cur = 1 - cur;
@@ -1295,11 +1408,11 @@ class MlpModel {
     bf16 *wb_ = nullptr;
     uint64_t *step_ = nullptr;
     float *lr_ = nullptr;
-    bf16 *X_ = nullptr, *Xt_ = nullptr, *dY_ = nullptr, *dYt_ = nullptr;
+    bf16 *X_ = nullptr, *dY_ = nullptr;
     float *Yt_ = nullptr, *Yp_ = nullptr;
     std::vector<bf16 *> G_;  // GELU'(Z) of each hidden layer (bf16, fragment order)
-    std::vector<bf16 *> Hs_, Ht_;
-    bf16 *dZ_[2] = {nullptr, nullptr}, *dZt_[2] = {nullptr, nullptr};
+    std::vector<bf16 *> Hs_;
+    bf16 *dZ_[2] = {nullptr, nullptr};
     bf16 *Xq_ = nullptr, *Hq_[2] = {nullptr, nullptr};
     float *xq_ = nullptr, *mean_ = nullptr, *spread_ = nullptr;
     float *xd_ = nullptr, *yd_ = nullptr;
