@@ -355,17 +355,22 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 //
 // A stage holds 32 k-rows x 128 columns of each operand, copied row by row from global memory
 // into LDS (global_load_lds_dwordx4: rows of 256 bytes, 16 lanes a row, coalesced); the 16-byte
-// chunk c of row r is stored at chunk c ^ 2 (r & 7). The MFMA wants 8 consecutive k of one column
+// chunk c of row r is stored at chunk c ^ s(r), s(r) = 2 (r & 3) + 8 ((r >> 3) & 1). The MFMA wants 8 consecutive k of one column
 // per lane, which ds_read_b64_tr_b16 delivers from this layout: within each 16 lanes, lane i
 // receives element i % 4 of the 8 bytes that lane 4 j + i / 4 addresses, as its element j
 // (j = 0..3). So lane L of group g addresses row 8 g + L / 4 (then 8 g + 4 + L / 4), columns
 // c0 + 4 (L % 4) .. + 3 of the fragment's 16 columns c0 .. c0 + 15, and receives column c0 + L at
-// k = 8 g .. 8 g + 3 (then 8 g + 4 .. 8 g + 7): the fragment the MFMA reads. The swizzle puts the
-// four rows a group reads in each instruction on distinct banks.
+// k = 8 g .. 8 g + 3 (then 8 g + 4 .. 8 g + 7): the fragment the MFMA reads. Rows are 256 bytes,
+// one pass over the 64 banks, so without the swizzle every row would use the same banks. With
+// it, the four rows a group reads in one instruction take four different pairs of chunks of
+// one 128-byte half of the bank space, and groups 2 h and 2 h + 1 (rows 8 apart), which the
+// LDS serves in the same cycle, take opposite halves: no conflicts (counters: half the LDS
+// cycles went to bank conflicts with the swizzle 2 (r & 7), which gave both groups the same
+// banks).
 typedef short shortx4 __attribute__((ext_vector_type(4)));
 constexpr int kTnStages = 3;
 constexpr int kTnLds = kTnStages * kStageElems * 2;  // bytes
-__device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 7); }
+__device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 3) + 8 * ((r >> 3) & 1); }
 
 // Eight ds_read_b64_tr_b16 at LDS byte addresses a[0..7], and the wait for them, in one block.
 // Written as asm because the builtin's reads are not tied to the stage they read, so the compiler
@@ -479,20 +484,18 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
     }
 #undef TMC_TN_ISSUE
-    // in each 16 x 16 tile the lane holds column n, rows mb .. mb + 3
+    // The operands are swapped (the tile is computed as B^T A, its transpose), so in each 16 x 16
+    // tile the lane holds row m = lane % 16 and four consecutive columns: one 16-byte store.
     float *C = g.Cf + h * g.sCf + split * g.sSplit;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        const long long row = static_cast<long long>(mblk + wm + 16 * i + (lane & 15)) * g.N;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = nblk + wn + 16 * j + (lane & 15);
-            const int mb = mblk + wm + 16 * i + (lane >> 4) * 4;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) C[static_cast<long long>(mb + q) * g.N + n] = acc[i][j][q];
-        }
+        for (int j = 0; j < 4; ++j)
+            *reinterpret_cast<floatx4 *>(C + row + nblk + wn + 16 * j + 4 * (lane >> 4)) = acc[i][j];
     }
 }
 
@@ -881,6 +884,8 @@ class MlpModel {
         set_lds<mlp::kFwdOut>();
         set_lds<mlp::kBwdData>();
         set_lds<mlp::kInferHidden>();
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_tn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kTnLds));
         H_ = c.heads;
         in_pad_ = pad128(c.inputs);
         k0_ = (c.inputs + mlp::kBK - 1) / mlp::kBK * mlp::kBK;

@@ -164,7 +164,8 @@ def bench_kernels(queries: int = 1_050_000, train_steps: int = 200, seed: int = 
         net.predict(x[:20000])  # warm-up
         _, _, _, sec = net.predict(x)
         per_step, per_query = net.flops()
-        net.set_data(x[:65536], rng.normal(0, 1, (65536, 202)).astype(np.float32))
+        n = min(queries, 65536)
+        net.set_data(x[:n], rng.normal(0, 1, (n, 202)).astype(np.float32))
         net.train(20)
         _, tsec = net.train(train_steps)
     return {"inference_queries": queries, "inference_device_s": sec,
