@@ -372,25 +372,36 @@ constexpr int kTnStages = 3;
 constexpr int kTnLds = kTnStages * kStageElems * 2;  // bytes
 __device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 3) + 8 * ((r >> 3) & 1); }
 
-// Eight ds_read_b64_tr_b16 at LDS byte addresses a[0..7], and the wait for them, in one block.
-// Written as asm because the builtin's reads are not tied to the stage they read, so the compiler
-// waits for every global-to-LDS copy in flight (the next stages' too) before them. The block
-// ends only when the reads have landed; its results are early-clobber, so no address register
-// is reused for a result while reads are still being issued.
-__device__ __forceinline__ void tr_read8(shortx4 (&o)[8], const uint32_t (&a)[8]) {
+// Sixteen ds_read_b64_tr_b16 at LDS byte addresses a[0..15], and the wait for them, in one
+// block. Written as asm because the builtin's reads are not tied to the stage they read, so the
+// compiler waits for every global-to-LDS copy in flight (the next stages' too) before them. The
+// block ends only when the reads have landed; its results are early-clobber, so no address
+// register is reused for a result while reads are still being issued.
+__device__ __forceinline__ void tr_read16(shortx4 (&o)[16], const uint32_t (&a)[16]) {
     asm volatile(
-        "ds_read_b64_tr_b16 %0, %8\n\t"
-        "ds_read_b64_tr_b16 %1, %9\n\t"
-        "ds_read_b64_tr_b16 %2, %10\n\t"
-        "ds_read_b64_tr_b16 %3, %11\n\t"
-        "ds_read_b64_tr_b16 %4, %12\n\t"
-        "ds_read_b64_tr_b16 %5, %13\n\t"
-        "ds_read_b64_tr_b16 %6, %14\n\t"
-        "ds_read_b64_tr_b16 %7, %15\n\t"
+        "ds_read_b64_tr_b16 %0, %16\n\t"
+        "ds_read_b64_tr_b16 %1, %17\n\t"
+        "ds_read_b64_tr_b16 %2, %18\n\t"
+        "ds_read_b64_tr_b16 %3, %19\n\t"
+        "ds_read_b64_tr_b16 %4, %20\n\t"
+        "ds_read_b64_tr_b16 %5, %21\n\t"
+        "ds_read_b64_tr_b16 %6, %22\n\t"
+        "ds_read_b64_tr_b16 %7, %23\n\t"
+        "ds_read_b64_tr_b16 %8, %24\n\t"
+        "ds_read_b64_tr_b16 %9, %25\n\t"
+        "ds_read_b64_tr_b16 %10, %26\n\t"
+        "ds_read_b64_tr_b16 %11, %27\n\t"
+        "ds_read_b64_tr_b16 %12, %28\n\t"
+        "ds_read_b64_tr_b16 %13, %29\n\t"
+        "ds_read_b64_tr_b16 %14, %30\n\t"
+        "ds_read_b64_tr_b16 %15, %31\n\t"
         "s_waitcnt lgkmcnt(0)"
         : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]),
-          "=&v"(o[6]), "=&v"(o[7])
-        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7])
+          "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9]), "=&v"(o[10]), "=&v"(o[11]),
+          "=&v"(o[12]), "=&v"(o[13]), "=&v"(o[14]), "=&v"(o[15])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
+          "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]), "v"(a[13]), "v"(a[14]),
+          "v"(a[15])
         : "memory");
 }
 
@@ -465,20 +476,19 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
         if (kt + kTnStages - 1 < nk) TMC_TN_ISSUE(kt + kTnStages - 1);
         const uint32_t abase = lds0 + static_cast<uint32_t>((kt % kTnStages) * kStageElems * 2);
         const uint32_t bbase = abase + 32 * 128 * 2;
-        uint32_t aa[8], ab[8];
+        uint32_t ad[16];  // A's eight reads, then B's
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            aa[u] = abase + aoff[u];
-            ab[u] = bbase + boff[u];
+            ad[u] = abase + aoff[u];
+            ad[8 + u] = bbase + boff[u];
         }
-        shortx4 ra[8], rb[8];
-        tr_read8(ra, aa);
-        tr_read8(rb, ab);
+        shortx4 rd[16];
+        tr_read16(rd, ad);
         bf16x8 fa[4], fb[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            fa[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(ra[i], ra[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
-            fb[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rb[i], rb[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+            fa[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[i], rd[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+            fb[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[8 + i], rd[12 + i], 0, 1, 2, 3, 4, 5, 6, 7));
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
