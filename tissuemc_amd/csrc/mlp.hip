@@ -410,11 +410,15 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int z = static_cast<int>(blockIdx.z);
+    // XCD-aware over the whole grid: workgroup L (launch order) runs on XCD L % 8; XCD k takes the
+    // contiguous logical workgroups [k W / 8, (k + 1) W / 8), i.e. whole (head, split) slices,
+    // whose tiles then share that slice's columns of dZ and H in one L2.
+    const int NX = static_cast<int>(gridDim.x), NY = static_cast<int>(gridDim.y);
+    const int T = NX * NY, W = T * static_cast<int>(gridDim.z);
+    int lid = static_cast<int>(blockIdx.x) + NX * (static_cast<int>(blockIdx.y) + NY * static_cast<int>(blockIdx.z));
+    if (W % 8 == 0) lid = (lid % 8) * (W / 8) + lid / 8;
+    const int z = lid / T, tile = lid % T;
     const int h = z / g.splits, split = z % g.splits;
-    const int NX = static_cast<int>(gridDim.x), T = NX * static_cast<int>(gridDim.y);
-    int tile = static_cast<int>(blockIdx.x) + NX * static_cast<int>(blockIdx.y);
-    if (T % 8 == 0) tile = (tile % 8) * (T / 8) + tile / 8;  // XCD-aware, as in gemm_nt
     const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 128;
     const int kspan = g.K / g.splits;
     const int kbeg = split * kspan;
