@@ -126,3 +126,34 @@ def test_set_params_round_trip(model):
     p = np.random.default_rng(1).normal(0, 0.1, (2, model.param_count)).astype(np.float32)
     model.set_params(p)
     assert np.array_equal(model.get_params(), p)
+
+
+def test_gradients_match_oracle_at_full_size(gpu):
+    """The training configuration itself (4 heads, 4 x 512 hidden, batch 4096): the weight
+    gradients then run split four ways along the batch, with the tile order dealt over the XCDs,
+    which the small network above does not reach. Heads 0 and 3 against the float64 oracle, to
+    the tolerances of the small case."""
+    from tissuemc_amd.mlp import Mlp
+
+    dims = (30, 512, 4, 202)
+    x, y = data(8192, seed=11)
+    w = np.linspace(0.5, 2.0, 202).astype(np.float32)
+    rows = np.random.default_rng(12).integers(0, 8192, 4096)
+    with Mlp(heads=4, inputs=30, hidden=512, depth=4, outputs=202, batch=4096, seed=5) as m:
+        m.set_data(x, y, w)
+        g, loss = m.gradients(rows)
+        params = m.get_params()
+    for h in (0, 3):
+        ref_loss, ref = M.loss_and_grad(params[h].astype(np.float64), x[rows].astype(np.float64),
+                                        y[rows].astype(np.float64), w.astype(np.float64), dims,
+                                        bf16=True)
+        assert loss[h] == pytest.approx(ref_loss, rel=2e-3)
+        cos = float(g[h] @ ref / (np.linalg.norm(g[h]) * np.linalg.norm(ref)))
+        assert cos > 0.9995
+        assert np.linalg.norm(g[h] - ref) <= 3e-2 * np.linalg.norm(ref)
+        k = 0
+        for o, i in M.shapes(*dims):
+            for n_el in (o * i, o):
+                a, b = g[h][k:k + n_el], ref[k:k + n_el]
+                assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b) + 1e-9
+                k += n_el
