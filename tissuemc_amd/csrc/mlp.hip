@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
 // cycles went to bank conflicts with the swizzle 2 (r & 7), which gave both groups the same
 // banks).
 typedef short shortx4 __attribute__((ext_vector_type(4)));
-constexpr int kTnStages = 3;
+constexpr int kTnStages = 4;
 constexpr int kTnLds = kTnStages * kStageElems * 2;  // bytes
 __device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 3) + 8 * ((r >> 3) & 1); }
 
@@ -473,12 +473,10 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
         __builtin_amdgcn_global_load_lds((global_cvoid *)(b1 + ko * g.ldb), (lds_void *)(Bs_ + 4 * 128), 16, 0, 0); \
     } while (0)
 
-    for (int st = 0; st < kTnStages - 1 && st < nk; ++st) TMC_TN_ISSUE(st);
-    for (int kt = 0; kt < nk; ++kt) {
-        const int ahead = nk - 1 - kt < kTnStages - 2 ? nk - 1 - kt : kTnStages - 2;
-        stage_sync_ahead<kTnStages - 2>(ahead);
-        if (kt + kTnStages - 1 < nk) TMC_TN_ISSUE(kt + kTnStages - 1);
-        const uint32_t abase = lds0 + static_cast<uint32_t>((kt % kTnStages) * kStageElems * 2);
+    // Stage kt's fragments are read while the MFMAs of stage kt - 1 run: the reads of the next
+    // stage are issued after this stage's MFMAs, into a second set of registers.
+    auto read_stage = [&](int st, bf16x8 (&fa)[4], bf16x8 (&fb)[4]) {
+        const uint32_t abase = lds0 + static_cast<uint32_t>((st % kTnStages) * kStageElems * 2);
         const uint32_t bbase = abase + 32 * 128 * 2;
         uint32_t ad[16];  // A's eight reads, then B's
 #pragma unroll
@@ -488,17 +486,40 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
         }
         shortx4 rd[16];
         tr_read16(rd, ad);
-        bf16x8 fa[4], fb[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             fa[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[i], rd[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
             fb[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[8 + i], rd[12 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+        }
+    };
+    // The ring holds stages kt .. kt + kTnStages - 1: stage kt's slot is refilled (with stage
+    // kt + kTnStages) once every wave has read stage kt, which the barrier before stage kt + 1's
+    // reads establishes.
+    for (int st = 0; st < kTnStages && st < nk; ++st) TMC_TN_ISSUE(st);
+    stage_sync_ahead<kTnStages - 1>(nk - 1 < kTnStages - 1 ? nk - 1 : kTnStages - 1);
+    bf16x8 fa[4], fb[4];
+    read_stage(0, fa, fb);
+    for (int kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) {
+            // stages issued so far: up to min(kt + kTnStages - 1, nk - 1); wait for stage kt + 1
+            const int ahead = nk - 2 - kt < kTnStages - 2 ? nk - 2 - kt : kTnStages - 2;
+            stage_sync_ahead<kTnStages - 2>(ahead);
+            if (kt + kTnStages < nk) TMC_TN_ISSUE(kt + kTnStages);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+        if (kt + 1 < nk) {
+            bf16x8 na[4], nb[4];
+            read_stage(kt + 1, na, nb);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fa[i] = na[i];
+                fb[i] = nb[i];
+            }
+        }
     }
 #undef TMC_TN_ISSUE
     // The operands are swapped (the tile is computed as B^T A, its transpose), so in each 16 x 16
