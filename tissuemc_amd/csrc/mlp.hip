@@ -834,9 +834,12 @@ namespace mlp {
 
 // Inputs for prediction: the same rows for every head, bf16, zero-padded.
 // Queries -> bf16 network inputs X [B x in_pad] (rows past `rows` and columns past `in` are
-// zero), shared by every head: one thread per 8 columns, one 16-byte store.
-__global__ void load_inputs(const float *xf, int rows, int in, int in_pad, int B, bf16 *X) {
-    const int per_row = in_pad / 8;
+// zero), shared by every head: one thread per 8 columns, one 16-byte store. Only the first
+// in_used columns (a multiple of 8: the columns the first layer multiplies) are written; the
+// others stay as allocated, zero.
+__global__ void load_inputs(const float *xf, int rows, int in, int in_used, int in_pad, int B,
+                            bf16 *X) {
+    const int per_row = in_used / 8;
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(B) * per_row) return;
     const int m = static_cast<int>(t / per_row), j0 = static_cast<int>(t % per_row) * 8;
@@ -1102,8 +1105,8 @@ class MlpModel {
             MLP_TRY(hipMemcpyAsync(xq_, x + r0 * cfg_.inputs, static_cast<size_t>(rows) * cfg_.inputs * 4,
                                    hipMemcpyHostToDevice, stream_));
             MLP_TRY(hipEventRecord(e0_, stream_));
-            hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (in_pad_ / 8) + 255) / 256), dim3(256), 0,
-                               stream_, xq_, rows, cfg_.inputs, in_pad_, Bq_, Xq_);
+            hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (k0_ / 8) + 255) / 256), dim3(256), 0,
+                               stream_, xq_, rows, cfg_.inputs, k0_, in_pad_, Bq_, Xq_);
             forward(Xq_, Bq_, Hq_, nullptr, true);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
