@@ -157,3 +157,28 @@ def test_gradients_match_oracle_at_full_size(gpu):
                 a, b = g[h][k:k + n_el], ref[k:k + n_el]
                 assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b) + 1e-9
                 k += n_el
+
+
+def test_forward_matches_oracle_at_full_width(gpu):
+    """Inference at the training configuration's width (4 x 512 hidden), where the hidden layers
+    run on 128 x 256 tiles (gemm_infer_wide), against the float64 oracle for heads 0 and 3, to
+    the small network's tolerances; and the ensemble mean and spread against the per-head
+    outputs."""
+    from tissuemc_amd.mlp import Mlp
+
+    dims = (30, 512, 4, 202)
+    x, _ = data(1000, seed=13)
+    with Mlp(heads=4, inputs=30, hidden=512, depth=4, outputs=202, batch=4096, seed=6) as m:
+        mean, spread, ph, _ = m.predict(x, per_head=True)
+        mean2, spread2, _, _ = m.predict(x)
+        params = m.get_params()
+    for h in (0, 3):
+        ref_bf, _ = M.forward(params[h].astype(np.float64), x.astype(np.float64), dims, bf16=True)
+        ref64, _ = M.forward(params[h].astype(np.float64), x.astype(np.float64), dims, bf16=False)
+        scale = np.abs(ref64).max()
+        assert np.max(np.abs(ph[h] - ref_bf)) <= 2e-2 * scale
+        assert np.linalg.norm(ph[h] - ref_bf) <= 2e-3 * np.linalg.norm(ref_bf)
+        assert np.linalg.norm(ph[h] - ref64) <= 2e-2 * np.linalg.norm(ref64)
+    assert np.allclose(mean, ph.mean(axis=0), rtol=1e-5, atol=1e-6)
+    assert np.allclose(spread, ph[:, :, :2].std(axis=0, ddof=1), rtol=1e-3, atol=1e-6)
+    assert np.array_equal(mean, mean2) and np.array_equal(spread, spread2)

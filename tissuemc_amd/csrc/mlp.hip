@@ -348,6 +348,115 @@ __global__ __launch_bounds__(256) void gemm_nt(const GemmArgs g) {
     }
 }
 
+// stage_sync for `ahead` stages of kPer loads each still allowed in flight, 0 <= ahead <= kMax
+template <int kMax, int kPer>
+__device__ __forceinline__ void stage_sync_ahead_n(int ahead) {
+    if constexpr (kMax == 0) {
+        stage_sync<0>();
+    } else {
+        if (ahead >= kMax) stage_sync<kPer * kMax>();
+        else stage_sync_ahead_n<kMax - 1, kPer>(ahead);
+    }
+}
+
+// The inference products on 128 x 256 tiles (N a multiple of 256): the loop and epilogues of
+// gemm_nt<kInferHidden> (bias in the accumulators, swapped operands, GELU, 8-byte bf16 stores)
+// and gemm_nt<kFwdOut> (16-byte fp32 stores), with each wave computing 64 x 128 (4 x 8 MFMA
+// tiles) instead of 64 x 64. A stage's MFMA work
+// per wave doubles against the same barrier and half again as many fragment reads (12 for 32
+// MFMAs, against 8 for 16). A stage holds 128 rows of A then 256 of B (24 KB), three stages:
+// two workgroups per CU.
+constexpr int kWideStages = 3;
+constexpr int kWideStageElems = (128 + 256) * kBK;
+constexpr int kWideLds = kWideStages * kWideStageElems * 2;  // bytes
+
+template <int kEpi>
+__global__ __launch_bounds__(256, 2) void gemm_infer_wide(const GemmArgs g) {
+    static_assert(kEpi == kInferHidden || kEpi == kFwdOut, "the swapped inference epilogues");
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int h = static_cast<int>(blockIdx.z);
+    const int NX = static_cast<int>(gridDim.x), T = NX * static_cast<int>(gridDim.y);
+    int tile = static_cast<int>(blockIdx.x) + NX * static_cast<int>(blockIdx.y);
+    if (T % 8 == 0) tile = (tile % 8) * (T / 8) + tile / 8;  // XCD-aware, as in gemm_nt
+    const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 256;
+    const int m0 = mblk + (wave >> 1) * 64;
+    const int n0 = nblk + (wave & 1) * 128;
+    const int nk = g.K / kBK;
+    const bf16 *A = g.A + h * g.sA;
+    const bf16 *B = g.B + h * g.sB;
+
+    // loads: wave w moves rows 32 w + 16 i + lane / 4 (i = 0, 1) of A and 64 w + 16 i + lane / 4
+    // (i = 0..3) of B; the lane's LDS slot is chunk lane % 4 of its row (rows of 64 bytes, the
+    // chunk swizzle of gemm_nt), so it fetches the chunk stored there
+    const int q4 = lane >> 2;
+    const int lc = ((lane & 3) ^ swz(q4)) * 8;  // swz depends on the row mod 16, = q4 here
+    const bf16 *a0 = A + static_cast<long long>(mblk + 32 * wave + q4) * g.lda + lc;
+    const bf16 *b0 = B + static_cast<long long>(nblk + 64 * wave + q4) * g.ldb + lc;
+    const long long a16 = 16LL * g.lda, b16 = 16LL * g.ldb;
+    // fragments: lane holds row (lane % 16), k chunk (lane / 16)
+    const int r = lane & 15, kq = lane >> 4;
+    const int ch = (kq ^ swz(r)) * 8;
+    const int arow = ((wave >> 1) * 64 + r) * kBK + ch, brow = ((wave & 1) * 128 + r) * kBK + ch;
+
+    floatx4 acc[4][8];
+    const int g4 = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const floatx4 init = *reinterpret_cast<const floatx4 *>(g.bias + h * g.sBias + n0 + g4 + 16 * j);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i][j] = init;
+    }
+
+#define TMC_WIDE_ISSUE(st)                                                                        \
+    do {                                                                                          \
+        bf16 *S_ = smem + ((st) % kWideStages) * kWideStageElems;                                 \
+        bf16 *As_ = S_ + 32 * wave * kBK, *Bs_ = S_ + 128 * kBK + 64 * wave * kBK;                \
+        const int ko = (st) * kBK;                                                                \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a0 + ko), (lds_void *)As_, 16, 0, 0);   \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(a0 + a16 + ko), (lds_void *)(As_ + 16 * kBK), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + ko), (lds_void *)Bs_, 16, 0, 0);   \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + b16 + ko), (lds_void *)(Bs_ + 16 * kBK), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + 2 * b16 + ko), (lds_void *)(Bs_ + 32 * kBK), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + 3 * b16 + ko), (lds_void *)(Bs_ + 48 * kBK), 16, 0, 0); \
+    } while (0)
+
+    for (int st = 0; st < kWideStages - 1 && st < nk; ++st) TMC_WIDE_ISSUE(st);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int ahead = nk - 1 - kt < kWideStages - 2 ? nk - 1 - kt : kWideStages - 2;
+        stage_sync_ahead_n<kWideStages - 2, 6>(ahead);
+        if (kt + kWideStages - 1 < nk) TMC_WIDE_ISSUE(kt + kWideStages - 1);
+        const bf16 *As = smem + (kt % kWideStages) * kWideStageElems, *Bs = As + 128 * kBK;
+        bf16x8 fa[4], fb[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const bf16x8 *>(As + arow + i * 16 * kBK);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fb[j] = *reinterpret_cast<const bf16x8 *>(Bs + brow + j * 16 * kBK);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+    }
+#undef TMC_WIDE_ISSUE
+    // the lane holds row m0 + 16 i + (lane % 16) and the four columns n0 + 16 j + 4 (lane / 16) ..
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const long long row = static_cast<long long>(m0 + 16 * i + r) * g.N + n0 + g4;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const floatx4 &zz = acc[i][j];
+            if constexpr (kEpi == kInferHidden)
+                *reinterpret_cast<uint2 *>(g.Cb + h * g.sCb + row + 16 * j) =
+                    uint2{pack_bf16x2(gelu(zz[0]), gelu(zz[1])), pack_bf16x2(gelu(zz[2]), gelu(zz[3]))};
+            else
+                *reinterpret_cast<floatx4 *>(g.Cf + h * g.sCf + row + 16 * j) = zz;
+        }
+    }
+}
+
 // The weight gradient without transposed copies: C[m, n] = sum_k A[k, m] B[k, n] with A = dZ
 // [batch x out] and B = H [batch x in] as the forward and backward products wrote them (K = the
 // batch, the row index of both), split along K across gridDim.z, fp32 partials.
@@ -924,6 +1033,10 @@ class MlpModel {
         set_lds<mlp::kInferHidden>();
         MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_tn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kTnLds));
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_infer_wide<mlp::kInferHidden>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kWideLds));
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_infer_wide<mlp::kFwdOut>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kWideLds));
         H_ = c.heads;
         in_pad_ = pad128(c.inputs);
         k0_ = (c.inputs + mlp::kBK - 1) / mlp::kBK * mlp::kBK;
@@ -1234,6 +1347,15 @@ class MlpModel {
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     mlp::lds_bytes_of(kEpi)));
     }
+    // The 128 x 256 inference kernel takes N a multiple of 256 (the others use gemm_nt).
+    static bool wide_ok(const mlp::GemmArgs &g) {
+        return g.N % 256 == 0 && g.M % 128 == 0 && g.K % mlp::kBK == 0;
+    }
+    template <int kEpi> void gemm_wide(const mlp::GemmArgs &g) {
+        hipLaunchKernelGGL((mlp::gemm_infer_wide<kEpi>), dim3(g.N / 256, g.M / 128, H_), dim3(256),
+                           mlp::kWideLds, stream_, g);
+        MLP_TRY(hipGetLastError());
+    }
     template <int kEpi> void gemm(mlp::GemmArgs g) {
         if (g.splits < 1) g.splits = 1;
         if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
@@ -1300,10 +1422,13 @@ class MlpModel {
             g.Gd = infer ? nullptr : Gbuf[l];
             g.Cb = infer ? Hbuf[l % 2] : Hbuf[l];
             g.sCb = BH;
-            if (infer)
+            if (infer && wide_ok(g)) {
+                gemm_wide<mlp::kInferHidden>(g);
+            } else if (infer) {
                 gemm<mlp::kInferHidden>(g);
-            else
+            } else {
                 gemm<mlp::kFwdHidden>(g);
+            }
             in = g.Cb;
             in_dim = hid_;
         }
@@ -1322,7 +1447,10 @@ class MlpModel {
         g.sBias = P_;
         g.Cf = Yp_;
         g.sCf = static_cast<long long>(Bn) * out_pad_;
-        gemm<mlp::kFwdOut>(g);
+        if (infer && wide_ok(g))
+            gemm_wide<mlp::kFwdOut>(g);
+        else
+            gemm<mlp::kFwdOut>(g);
     }
 
     // One training step (gather, forward, loss, backward, and Adam if `update`).
