@@ -195,6 +195,8 @@ typedef struct tmc_tune_report {
 
 /* Single runs: copies *opts into *tuned with waves_per_group, groups and (with TMC_FLAG_ARZ) the
  * A(r,z) tile set to the fastest candidate; the launch-shape fields of *opts are not used.
+ * Candidates are timed in a fixed order, the default shape first, and one replaces the fastest
+ * so far only if it is more than 3% faster (pilot timings scatter by about that much).
  * pilot_fraction is in (0, 1]. report may be NULL. */
 int tmc_autotune(const tmc_spec *spec, const tmc_opts *opts, double pilot_fraction,
                  tmc_opts *tuned, tmc_tune_report *report);

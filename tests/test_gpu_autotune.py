@@ -21,8 +21,14 @@ def check_report(rep):
     tried = rep["tried"]
     assert len(tried) >= 3
     assert 0 <= rep["best"] < len(tried)
-    best = tried[rep["best"]]["kernel_seconds"]
-    assert best == min(t["kernel_seconds"] for t in tried)
+    # in the order tried, a candidate replaces the best so far only if it is more than 3% faster;
+    # the choice is then within 3% of the fastest pilot
+    b = 0
+    for k in range(1, len(tried)):
+        if tried[k]["kernel_seconds"] < tried[b]["kernel_seconds"] * 0.97:
+            b = k
+    assert rep["best"] == b
+    assert tried[b]["kernel_seconds"] * 0.97 <= min(t["kernel_seconds"] for t in tried)
     assert all(t["kernel_seconds"] > 0 for t in tried)
     assert len({(t["waves_per_group"], t["groups"]) for t in tried}) == len(tried)
 
