@@ -69,3 +69,20 @@ def test_digest_prints_ratios():
     b = {"duration_us": 100.0, "counters": {"SQ_WAVES": 4096.0}, "derived": {"valu_per_wave": 150.0}}
     text = "\n".join(P.digest(a, b))
     assert "0.500" in text and "SQ_WAVES" in text and "1.000" in text
+
+
+def test_run_options_reach_the_engine_command():
+    import argparse
+
+    args = argparse.Namespace(mci=Path("tissuemc_amd/configs/three_layer_c2.mci"), photons=1000,
+                              batches=10, seed=3, arz=True, angles=True, aggregate=True,
+                              time=["200", "0.005"], no_profiles=False)
+    prog = P.engine_program(args)
+    assert prog[1] == "run" and prog[2].endswith("three_layer_c2.mci")
+    assert prog[3:9] == ["--photons", "1000", "--batches", "10", "--seed", "3"]
+    assert {"--arz", "--angles", "--aggregate"} <= set(prog)
+    i = prog.index("--time")
+    assert prog[i + 1:i + 3] == ["200", "0.005"]
+    args.time, args.arz, args.angles, args.aggregate, args.no_profiles = None, False, False, False, True
+    prog = P.engine_program(args)
+    assert "--time" not in prog and "--arz" not in prog and prog[-1] == "--no-profiles"

@@ -167,6 +167,17 @@ def digest(a: dict, b: dict) -> list[str]:
     return lines
 
 
+def engine_program(args) -> list[str]:
+    """The `tissuemc run` command line that a `run` subcommand profiles."""
+    program = [str(CLI), "run", str(args.mci.resolve()), "--photons", str(args.photons),
+               "--batches", str(args.batches), "--seed", str(args.seed)]
+    program += ["--arz"] * args.arz + ["--angles"] * args.angles + ["--aggregate"] * args.aggregate
+    if args.time:
+        program += ["--time", *args.time]
+    program += ["--no-profiles"] * args.no_profiles
+    return program
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(prog="python -m tissuemc_amd.profile",
                                  description=__doc__.split("\n\n")[0])
@@ -198,12 +209,7 @@ def main(argv: list[str] | None = None) -> int:
 
     if not CLI.exists():
         raise SystemExit(f"the engine's host program is not built ({CLI}); run make -C tissuemc_amd")
-    program = [str(CLI), "run", str(args.mci.resolve()), "--photons", str(args.photons),
-               "--batches", str(args.batches), "--seed", str(args.seed)]
-    program += ["--arz"] * args.arz + ["--angles"] * args.angles + ["--aggregate"] * args.aggregate
-    if args.time:
-        program += ["--time", *args.time]
-    program += ["--no-profiles"] * args.no_profiles
+    program = engine_program(args)
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         res = profile(program, args.kernel, Path(tmp), args.fetch_scale, args.write_scale)
     args.output.parent.mkdir(parents=True, exist_ok=True)
