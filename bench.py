@@ -60,7 +60,7 @@ def gate(engine, spec: dict) -> None:
 
     n = 100_000 if spec["layers"][0]["d"] < 1e3 else 20_000
     g = engine.run(spec, photons=n, batches=20, seed=97, arz=True)
-    o = oracle.run(spec, photons=n, batches=20, seed=97, precision="float32")
+    o = oracle.run(spec, photons=n, batches=20, seed=97, precision="float32", arz=True)
     bad = [x.line() for x in cross_engine.compare_runs(o, g) if not x.passed]
     if bad or g["anomalies"]:
         raise SystemExit("bench: validation gate failed, not timing:\n" + "\n".join(bad))
