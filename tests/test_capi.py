@@ -196,3 +196,59 @@ def test_time_and_aggregation_options_are_checked_before_any_device_work():
     plan = engine.C.c_void_p()
     assert L.tmc_batched_create(4, engine.C.byref(o), engine.C.byref(plan)) == engine.TMC_ERR_CONFIG
     assert "single-run options" in L.tmc_last_error().decode()
+
+
+# Inputs the reader must reject, each with the message the reference's own parser gives.
+BAD_INPUTS = [
+    "",
+    "# only a comment\n\n",
+    "2.0\n1\n",
+    "1.0\n0\n",
+    "1.0\n1.5\n",
+    "1.0\n1\ncomp.mco\n",
+    "1.0\n1\ncomp.mco B\n",
+    "1.0\n1\ncomp.mco A\n",
+    "1.0\n1\ncomp.mco A\n100.0\n",
+    "1.0\n1\ncomp.mco A\n0\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 x\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 inf\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 1e999\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 0.1\n10 10\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 0.1\n10 0 1\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 0.1\n10 10 2000000\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 0.1\n10 10 1\n17\n",
+    "1.0\n1\ncomp.mco A\n100\n0.1 0.1\n10 10 1\n1\n1.0\n1.4 1 10 0.9\n",
+]
+
+
+@pytest.mark.parametrize("mutation", [
+    ("1.0 # file version", "2.0 # file version"), ("comp.mco A", "comp.mco B"),
+    ("1000000 #", "1000000.5 #"), ("1.37 1 10 0 0.1", "1.37 1 10 0"),
+    ("1.37 1 100 0.90 0.1", "1.37 1 100 abc 0.1"), ("1.37 1 100 0.90 0.1", "1.37 1 100 1.0 0.1"),
+    ("1.37 1 100 0.90 0.1", "1.37 -1 100 0.9 0.1"), ("1.37 1 100 0.90 0.1", "1.37 1 100 0.9 0"),
+    ("1.37 1 100 0.90 0.1", "0.9 1 100 0.9 0.1"), ("1.37 1 100 0.90 0.1", "1.37 0 0 0 0.1"),
+    ("1.0 # n for medium above.", "0.5 # n for medium above."),
+    ("3 # No. of layers", "17 # No. of layers"), ("1 # number of runs", "2 # number of runs"),
+    ("1.0 # n for medium below.\n", "1.0 # n for medium below.\nextra line\n"),
+] + [(None, t) for t in BAD_INPUTS])
+def test_error_messages_match_the_reference_parser(tmp_path, mutation):
+    """Same input, same message: this reader's error text against the reference engine's own
+    parser (its CLI prints "tissuemc: error: <message>"), built from its sources where they are
+    present."""
+    import oracle
+
+    if not oracle.ref_available():
+        pytest.skip("the reference engine is not built here")
+    old, new = mutation
+    text = new if old is None else mutate(old, new)
+    path = tmp_path / "input.mci"
+    path.write_text(text)
+    ref = subprocess.run([str(oracle.REF_BIN), "run", str(path), "--photons", "1"],
+                         capture_output=True, text=True)
+    assert ref.returncode == 1, ref.stderr
+    want = ref.stderr.strip().removeprefix("tissuemc: error: ")
+    with pytest.raises(engine.ConfigError) as err:
+        engine.parse_mci(path=path)
+    assert str(err.value) == want
+    ours = cli("run", str(path))
+    assert ours.returncode == 1 and ours.stderr.strip() == ref.stderr.strip()
