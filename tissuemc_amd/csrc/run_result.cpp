@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <numeric>
 
 namespace tmc {
 
@@ -16,75 +17,72 @@ constexpr double kUnit = 0x1p-32;  // weight of one tally unit
 }
 
 double absorbed_sum(const BatchResult &b) {
-    double s = 0;
-    for (double a : b.absorbed) s += a;
-    return s;
+    return std::accumulate(b.absorbed.begin(), b.absorbed.end(), 0.0);
 }
 
-template <class F> Estimate estimate_of(const std::vector<BatchResult> &bs, F f) {
-    std::vector<double> sums;
-    std::vector<uint64_t> ph;
-    for (const auto &b : bs) {
-        sums.push_back(f(b));
-        ph.push_back(b.photons);
-    }
-    return batch_estimate(sums, ph);
+// Batch-means estimate of one per-batch quantity over a run's batches.
+template <class F> Estimate estimate_over(const std::vector<BatchResult> &bs, F value) {
+    BatchMeans m;
+    for (const BatchResult &b : bs) m.add(value(b), b.photons);
+    return m.estimate();
+}
+
+template <class F> uint64_t count_over(const std::vector<BatchResult> &bs, F field) {
+    return std::accumulate(bs.begin(), bs.end(), uint64_t{0},
+                           [&](uint64_t n, const BatchResult &b) { return n + field(b); });
 }
 
 }  // namespace
 
-Estimate batch_estimate(const std::vector<double> &sums, const std::vector<uint64_t> &photons) {
-    double total = 0;
-    uint64_t n_ph = 0;
-    for (size_t i = 0; i < sums.size(); ++i) {
-        total += sums[i];
-        n_ph += photons[i];
-    }
+void BatchMeans::add(double sum, uint64_t photons) {
+    total_ += sum;
+    photons_ += photons;
+    // Welford's update of the mean and the sum of squared deviations of the per-batch means
+    const double x = sum / static_cast<double>(photons);
+    ++n_;
+    const double d = x - center_;
+    center_ += d / static_cast<double>(n_);
+    m2_ += d * (x - center_);
+}
+
+Estimate BatchMeans::estimate() const {
     Estimate e;
-    e.mean = total / static_cast<double>(n_ph);
-    const size_t n = sums.size();
-    if (n < 2) {
-        e.se = std::nan("");
-        return e;
-    }
-    double center = 0;
-    for (size_t i = 0; i < n; ++i) center += sums[i] / static_cast<double>(photons[i]);
-    center /= static_cast<double>(n);
-    double ss = 0;
-    for (size_t i = 0; i < n; ++i) {
-        const double d = sums[i] / static_cast<double>(photons[i]) - center;
-        ss += d * d;
-    }
-    e.se = std::sqrt(ss / static_cast<double>(n - 1) / static_cast<double>(n));
+    e.mean = total_ / static_cast<double>(photons_);
+    e.se = n_ < 2 ? std::nan("")
+                  : std::sqrt(m2_ / static_cast<double>(n_ - 1) / static_cast<double>(n_));
     return e;
 }
 
+Estimate batch_estimate(const std::vector<double> &sums, const std::vector<uint64_t> &photons) {
+    BatchMeans m;
+    for (size_t i = 0; i < sums.size(); ++i) m.add(sums[i], photons[i]);
+    return m.estimate();
+}
+
 uint64_t RunResult::photons() const {
-    uint64_t n = 0;
-    for (const auto &b : batches) n += b.photons;
-    return n;
+    return count_over(batches, [](const BatchResult &b) { return b.photons; });
 }
 uint64_t RunResult::events() const {
-    uint64_t n = 0;
-    for (const auto &b : batches) n += b.events;
-    return n;
+    return count_over(batches, [](const BatchResult &b) { return b.events; });
 }
 uint64_t RunResult::anomalies() const {
-    uint64_t n = 0;
-    for (const auto &b : batches) n += b.anomalies;
-    return n;
+    return count_over(batches, [](const BatchResult &b) { return b.anomalies; });
 }
 uint64_t RunResult::boundary_steps() const {
-    uint64_t n = 0;
-    for (const auto &b : batches) n += b.boundary_steps;
-    return n;
+    return count_over(batches, [](const BatchResult &b) { return b.boundary_steps; });
 }
-Estimate RunResult::rd() const { return estimate_of(batches, [](const BatchResult &b) { return b.rd; }); }
-Estimate RunResult::tt() const { return estimate_of(batches, [](const BatchResult &b) { return b.tt; }); }
-Estimate RunResult::absorbed() const { return estimate_of(batches, absorbed_sum); }
+Estimate RunResult::rd() const {
+    return estimate_over(batches, [](const BatchResult &b) { return b.rd; });
+}
+Estimate RunResult::tt() const {
+    return estimate_over(batches, [](const BatchResult &b) { return b.tt; });
+}
+Estimate RunResult::absorbed() const { return estimate_over(batches, absorbed_sum); }
 Estimate RunResult::conservation_residual() const {
-    Estimate e = estimate_of(batches, [](const BatchResult &b) { return b.rd + b.tt + absorbed_sum(b); });
-    e.mean = e.mean + specular - 1.0;
+    // every launched photon's weight ends as specular, diffuse reflectance, transmittance or
+    // absorption: the residual estimates (sum of the four) - 1
+    Estimate e = estimate_over(batches, [](const BatchResult &b) { return b.rd + b.tt + absorbed_sum(b); });
+    e.mean += specular - 1.0;
     return e;
 }
 
@@ -214,277 +212,95 @@ RunResult run_hip(const tmc_spec &spec_in, const tmc_opts &opts) {
 }
 
 // ------------------------------------------------------------------ JSON
+//
+// The document of one run: the reference's keys in its order (src/io/json_out.cpp:164-306, the
+// schema users read), then this engine's additions.
 
 namespace {
 
-class Json {
-  public:
-    explicit Json(std::ostream &o) : o_(o) {}
-    void raw(const char *s) { o_ << s; }
-    void str(const std::string &s) {
-        o_ << '"';
-        for (char c : s) {
-            if (c == '"' || c == '\\') {
-                o_ << '\\' << c;
-            } else if (static_cast<unsigned char>(c) < 0x20) {
-                char buf[8];
-                std::snprintf(buf, sizeof buf, "\\u%04x", static_cast<unsigned>(c));
-                o_ << buf;
-            } else {
-                o_ << c;
-            }
-        }
-        o_ << '"';
-    }
-    // 17 significant digits round-trip a double; JSON has no NaN, so it is written as null.
-    void num(double v) {
-        if (!std::isfinite(v)) {
-            o_ << "null";
-            return;
-        }
-        char buf[40];
-        std::snprintf(buf, sizeof buf, "%.17g", v);
-        o_ << buf;
-    }
-    void integer(uint64_t v) { o_ << v; }
-    void key(const char *k) {
-        str(k);
-        o_ << ':';
-    }
-    void array(const std::vector<double> &v) {
-        o_ << '[';
-        for (size_t i = 0; i < v.size(); ++i) {
-            if (i) o_ << ',';
-            num(v[i]);
-        }
-        o_ << ']';
-    }
-    void estimate(const Estimate &e) {
-        o_ << '{';
-        key("mean");
-        num(e.mean);
-        o_ << ',';
-        key("se");
-        num(e.se);
-        o_ << '}';
-    }
+json::Value estimate_doc(const Estimate &e) {
+    return json::Value::record().set("mean", e.mean).set("se", e.se);
+}
 
-  private:
-    std::ostream &o_;
-};
+json::Value layer_doc(const tmc_layer &L) {
+    return json::Value::record()
+        .set("n", L.n).set("mua", L.mua).set("mus", L.mus).set("g", L.g).set("d", L.d);
+}
+
+json::Value batch_doc(const BatchResult &b) {
+    return json::Value::record()
+        .set("first_photon", b.first_photon)
+        .set("photons", b.photons)
+        .set("rd", b.rd)
+        .set("tt", b.tt)
+        .set("absorbed", json::Value::list_of(b.absorbed))
+        .set("events", b.events)
+        .set("anomalies", b.anomalies)
+        .set("boundary_steps", b.boundary_steps);
+}
 
 }  // namespace
 
-void write_json(const RunResult &run, std::ostream &out) {
-    Json j(out);
+json::Value run_doc(const RunResult &run) {
+    using json::Value;
     const tmc_spec &s = run.spec;
-    j.raw("{");
-    j.key("tissuemc_version");
-    j.str("0.1.0");
-    j.raw(",");
-    j.key("engine");
-    j.str(run.engine);
-    j.raw(",");
-    j.key("precision");
-    j.str(run.precision);
-    j.raw(",");
-    j.key("seed");
-    j.integer(run.seed);
-    j.raw(",");
-    j.key("threads");
-    j.integer(static_cast<uint64_t>(run.threads));
-    j.raw(",");
-    j.key("photons");
-    j.integer(run.photons());
-    j.raw(",");
-    j.key("output_name");
-    j.str(s.output_name);
-    j.raw(",");
-    j.key("n_above");
-    j.num(s.n_above);
-    j.raw(",");
-    j.key("n_below");
-    j.num(s.n_below);
-    j.raw(",");
-    j.key("layers");
-    j.raw("[");
-    for (int i = 0; i < s.num_layers; ++i) {
-        const tmc_layer &L = s.layers[i];
-        j.raw(i ? ",{" : "{");
-        j.key("n");
-        j.num(L.n);
-        j.raw(",");
-        j.key("mua");
-        j.num(L.mua);
-        j.raw(",");
-        j.key("mus");
-        j.num(L.mus);
-        j.raw(",");
-        j.key("g");
-        j.num(L.g);
-        j.raw(",");
-        j.key("d");
-        j.num(L.d);
-        j.raw("}");
-    }
-    j.raw("],");
-    j.key("grid");
-    j.raw("{");
-    j.key("dz");
-    j.num(s.dz);
-    j.raw(",");
-    j.key("dr");
-    j.num(s.dr);
-    j.raw(",");
-    j.key("nz");
-    j.integer(static_cast<uint64_t>(s.nz));
-    j.raw(",");
-    j.key("nr");
-    j.integer(static_cast<uint64_t>(s.nr));
-    j.raw("},");
-    j.key("specular");
-    j.num(run.specular);
-    j.raw(",");
-    j.key("wall_seconds");
-    j.num(run.wall_seconds);
-    j.raw(",");
-    j.key("kernel_seconds");
-    j.num(run.kernel_seconds);
-    j.raw(",");
-    j.key("device");
-    j.str(run.device);
-    j.raw(",");
-    j.key("tally_precision");
-    j.str(run.tally_precision);
-    j.raw(",");
-    j.key("rung");
-    j.integer(static_cast<uint64_t>(run.rung));
-    j.raw(",");
-    j.key("profiles");
-    j.raw(run.profiles ? "true" : "false");
-    j.raw(",");
-    j.key("events");
-    j.integer(run.events());
-    j.raw(",");
-    j.key("anomalies");
-    j.integer(run.anomalies());
-    j.raw(",");
-    j.key("boundary_steps");
-    j.integer(run.boundary_steps());
-    j.raw(",");
-    j.key("launch");
-    j.raw("{");
-    j.key("groups");
-    j.integer(static_cast<uint64_t>(run.groups));
-    j.raw(",");
-    j.key("waves_per_group");
-    j.integer(static_cast<uint64_t>(run.waves_per_group));
-    j.raw(",");
-    j.key("lds_bytes");
-    j.integer(static_cast<uint64_t>(run.lds_bytes));
-    j.raw(",");
-    j.key("chunk");
-    j.integer(static_cast<uint64_t>(run.chunk));
-    j.raw(",");
-    j.key("arz_tile");
-    j.raw("[");
-    j.integer(static_cast<uint64_t>(run.arz_tile_z));
-    j.raw(",");
-    j.integer(static_cast<uint64_t>(run.arz_tile_r));
-    j.raw("]}");
-    j.raw(",");
-    j.key("summary");
-    j.raw("{");
-    j.key("rd");
-    j.estimate(run.rd());
-    j.raw(",");
-    j.key("tt");
-    j.estimate(run.tt());
-    j.raw(",");
-    j.key("absorbed");
-    j.estimate(run.absorbed());
-    j.raw(",");
-    j.key("conservation_residual");
-    j.estimate(run.conservation_residual());
-    j.raw("},");
-    j.key("batches");
-    j.raw("[");
-    for (size_t i = 0; i < run.batches.size(); ++i) {
-        const BatchResult &b = run.batches[i];
-        j.raw(i ? ",{" : "{");
-        j.key("first_photon");
-        j.integer(b.first_photon);
-        j.raw(",");
-        j.key("photons");
-        j.integer(b.photons);
-        j.raw(",");
-        j.key("rd");
-        j.num(b.rd);
-        j.raw(",");
-        j.key("tt");
-        j.num(b.tt);
-        j.raw(",");
-        j.key("absorbed");
-        j.array(b.absorbed);
-        j.raw(",");
-        j.key("events");
-        j.integer(b.events);
-        j.raw(",");
-        j.key("anomalies");
-        j.integer(b.anomalies);
-        j.raw(",");
-        j.key("boundary_steps");
-        j.integer(b.boundary_steps);
-        j.raw("}");
-    }
-    j.raw("],");
-    j.key("a_z");
-    j.array(run.a_z);
-    j.raw(",");
-    j.key("rd_r");
-    j.array(run.rd_r);
-    j.raw(",");
-    j.key("t_r");
-    j.array(run.t_r);
-    j.raw(",");
-    j.key("profile_se");
-    j.raw("{");
-    j.key("a_z");
-    j.array(run.a_z_se);
-    j.raw(",");
-    j.key("rd_r");
-    j.array(run.rd_r_se);
-    j.raw(",");
-    j.key("t_r");
-    j.array(run.t_r_se);
-    j.raw("}");
-    if (!run.a_rz.empty()) {
-        j.raw(",");
-        j.key("a_rz");
-        j.array(run.a_rz);
-    }
-    if (!run.rd_t.empty()) {
-        j.raw(",");
-        j.key("time_dt_ns");
-        j.num(run.dt_ns);
-        j.raw(",");
-        j.key("rd_t");
-        j.array(run.rd_t);
-        j.raw(",");
-        j.key("t_t");
-        j.array(run.t_t);
-    }
-    if (!run.rd_a.empty()) {
-        const std::pair<const char *, const std::vector<double> *> angle[] = {
-            {"rd_a", &run.rd_a}, {"t_a", &run.t_a}, {"rd_ra", &run.rd_ra}, {"t_ra", &run.t_ra}};
-        for (const auto &kv : angle) {
-            j.raw(",");
-            j.key(kv.first);
-            j.array(*kv.second);
-        }
-    }
-    j.raw("}");
+    Value layers = Value::list();
+    for (int i = 0; i < s.num_layers; ++i) layers.push(layer_doc(s.layers[i]));
+    Value batches = Value::list();
+    for (const BatchResult &b : run.batches) batches.push(batch_doc(b));
+    Value doc = Value::record();
+    doc.set("tissuemc_version", "0.1.0")
+        .set("engine", run.engine)
+        .set("precision", run.precision)
+        .set("seed", run.seed)
+        .set("threads", run.threads)
+        .set("photons", run.photons())
+        .set("output_name", std::string(s.output_name))
+        .set("n_above", s.n_above)
+        .set("n_below", s.n_below)
+        .set("layers", std::move(layers))
+        .set("grid", Value::record().set("dz", s.dz).set("dr", s.dr).set("nz", s.nz).set("nr", s.nr))
+        .set("specular", run.specular)
+        .set("wall_seconds", run.wall_seconds)
+        .set("kernel_seconds", run.kernel_seconds)
+        .set("device", run.device)
+        .set("tally_precision", run.tally_precision)
+        .set("rung", run.rung)
+        .set("profiles", run.profiles)
+        .set("events", run.events())
+        .set("anomalies", run.anomalies())
+        .set("boundary_steps", run.boundary_steps())
+        .set("launch", Value::record()
+                           .set("groups", run.groups)
+                           .set("waves_per_group", run.waves_per_group)
+                           .set("lds_bytes", run.lds_bytes)
+                           .set("chunk", run.chunk)
+                           .set("arz_tile", Value::list().push(run.arz_tile_z).push(run.arz_tile_r)))
+        .set("summary", Value::record()
+                            .set("rd", estimate_doc(run.rd()))
+                            .set("tt", estimate_doc(run.tt()))
+                            .set("absorbed", estimate_doc(run.absorbed()))
+                            .set("conservation_residual", estimate_doc(run.conservation_residual())))
+        .set("batches", std::move(batches))
+        .set("a_z", Value::list_of(run.a_z))
+        .set("rd_r", Value::list_of(run.rd_r))
+        .set("t_r", Value::list_of(run.t_r))
+        .set("profile_se", Value::record()
+                               .set("a_z", Value::list_of(run.a_z_se))
+                               .set("rd_r", Value::list_of(run.rd_r_se))
+                               .set("t_r", Value::list_of(run.t_r_se)));
+    if (!run.a_rz.empty()) doc.set("a_rz", Value::list_of(run.a_rz));
+    if (!run.rd_t.empty())
+        doc.set("time_dt_ns", run.dt_ns).set("rd_t", Value::list_of(run.rd_t)).set("t_t", Value::list_of(run.t_t));
+    if (!run.rd_a.empty())
+        doc.set("rd_a", Value::list_of(run.rd_a))
+            .set("t_a", Value::list_of(run.t_a))
+            .set("rd_ra", Value::list_of(run.rd_ra))
+            .set("t_ra", Value::list_of(run.t_ra));
+    return doc;
 }
+
+void write_json(const RunResult &run, std::ostream &out) { run_doc(run).write(out); }
 
 // ------------------------------------------------------------------ MCML .mco
 

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "json_doc.h"
 #include "tissuemc_hip.h"
 
 namespace tmc {
@@ -69,7 +70,20 @@ struct RunResult {
     Estimate conservation_residual() const;
 };
 
-// Batch-means estimate of E[value per photon] (the reference's json_out.cpp:37-64).
+// Batch-means estimate of E[value per photon] from per-batch sums (the estimator of the
+// reference's json_out.cpp:37-64): mean = sum of the sums / sum of the photons; standard error =
+// sample standard deviation of the per-batch means / sqrt(batches), NaN with fewer than two
+// batches. The spread is accumulated in one pass with Welford's update.
+class BatchMeans {
+  public:
+    void add(double sum, uint64_t photons);
+    Estimate estimate() const;
+
+  private:
+    double total_ = 0.0, center_ = 0.0, m2_ = 0.0;
+    uint64_t photons_ = 0, n_ = 0;
+};
+
 Estimate batch_estimate(const std::vector<double> &sums, const std::vector<uint64_t> &photons);
 
 // Run one spec on the GPU through the C ABI; throws ConfigError / std::runtime_error.
@@ -78,6 +92,9 @@ RunResult run_hip(const tmc_spec &spec, const tmc_opts &opts);
 // Parse an .mci file through the C ABI; throws ConfigError naming the file and line.
 std::vector<tmc_spec> parse_mci_file(const std::string &path);
 
+// The JSON document of one run (the reference's schema plus this engine's additions) and its
+// text.
+json::Value run_doc(const RunResult &run);
 void write_json(const RunResult &run, std::ostream &out);
 
 // MCML's ASCII output (.mco, "A1" format): the echoed input, then RAT, A_l, A_z, Rd_r, Rd_a,
