@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -67,43 +68,86 @@ def gate(engine, spec: dict) -> None:
     log(f"bench: gate passed ({n} photons identical-photon check)")
 
 
-def cpu_baseline(mci: Path, full_photons: int, batches: int, target_s: float = 12.0) -> dict:
-    """The reference CPU engine on a bounded sample, all host cores; or the oracle if the
-    reference build is absent."""
+def host_info() -> dict:
+    """CPU model and counts of the host the CPU baseline ran on (fields that cannot be read are
+    left out)."""
+    info = {"logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    cpuinfo = Path("/proc/cpuinfo")
+    if cpuinfo.exists():
+        for line in cpuinfo.read_text().splitlines():
+            if line.startswith("model name"):
+                info["cpu"] = line.split(":", 1)[1].strip()
+                break
+    return info
+
+
+def quartiles(values: list[float]) -> dict:
+    q = statistics.quantiles(values, n=4, method="inclusive")
+    return {"median": statistics.median(values), "q1": q[0], "q3": q[2], "iqr": q[2] - q[0],
+            "runs": values}
+
+
+def cpu_baseline(mci: Path, full_photons: int, batches: int, repeats: int = 5,
+                 per_run_s: float = 5.0) -> dict:
+    """The reference CPU engine (float64), timed like the reference's own benchmark
+    (python/tissuemc_analysis/bench.py:143-187): one warm-up run, then `repeats` timed runs of
+    the same sample, median and interquartile range of the engine's own wall clock; the answer
+    must be identical on every run and at one thread, or the baseline is refused. The oracle's
+    restatement stands in where the reference build is absent.
+
+    Each run is about `per_run_s` seconds (the reference's own runs are >= 30 s): the whole
+    CPU leg stays near 30 s so that the benchmark finishes within a few minutes; photons/s does
+    not depend on the sample size once it is far above threads x batches."""
     import oracle
 
     # the box's CPU share for one GPU is 16 threads, whatever the host reports
-    threads = max(1, min(os.cpu_count() or 1, 16, batches))
-    sample = 200_000
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    spec = oracle.parse_mci_text(mci.read_text())[0]
     if oracle.ref_available():
-        def timed(n: int) -> float:
-            with tempfile.TemporaryDirectory() as tmp:
-                t0 = time.perf_counter()
-                oracle.run_ref(mci, photons=n, batches=batches, seed=1, threads=threads,
-                               out_json=Path(tmp) / "r.json")
-                return time.perf_counter() - t0
         kind = "reference"
-    else:
-        spec = oracle.parse_mci_text(mci.read_text())[0]
 
-        def timed(n: int) -> float:
-            t0 = time.perf_counter()
-            oracle.run(spec, photons=n, batches=batches, seed=1, threads=threads)
-            return time.perf_counter() - t0
+        def run(n: int, t: int) -> tuple[float, tuple]:
+            with tempfile.TemporaryDirectory() as tmp:
+                r = oracle.run_ref(mci, photons=n, batches=batches, seed=1, threads=t,
+                                   out_json=Path(tmp) / "r.json")
+            return r["wall_seconds"], (r["events"], r["summary"]["rd"]["mean"],
+                                       r["summary"]["tt"]["mean"], tuple(r["a_z"]))
+    else:
         kind = "port"
-    t = timed(sample)  # calibration, also the warm-up
-    n = max(batches, min(full_photons, int(sample * target_s / max(t, 1e-3))))
-    t = timed(n)
-    return {"value": n / t, "unit": "photons/s", "cores": threads, "kind": kind,
+
+        def run(n: int, t: int) -> tuple[float, tuple]:
+            r = oracle.run(spec, photons=n, batches=batches, seed=1, threads=t)
+            return r["wall_seconds"], (r["events"], tuple(b["rd"] for b in r["batches"]),
+                                       tuple(b["tt"] for b in r["batches"]), tuple(r["a_z"]))
+
+    # the same answer at one thread and at `threads` (small sample)
+    small = max(batches, 20_000)
+    if run(small, 1)[1] != run(small, threads)[1]:
+        raise SystemExit("bench: CPU baseline refused: results differ between thread counts")
+    t_small, _ = run(small * 10, threads)  # calibration
+    n = max(batches, min(full_photons, int(small * 10 * per_run_s / max(t_small, 1e-3))))
+    _, want = run(n, threads)  # warm-up
+    walls = []
+    for _ in range(repeats):
+        w, got = run(n, threads)
+        if got != want:
+            raise SystemExit("bench: CPU baseline refused: repeats of one run differ")
+        walls.append(w)
+    q = quartiles(walls)
+    return {"value": n / q["median"], "unit": "photons/s", "cores": threads, "kind": kind,
+            "events_per_s": want[0] / q["median"],
+            "wall_seconds": q, "host": host_info(),
             "sample": f"{n} photons of the same workload (seed 1, {batches} batches), float64, "
-                      f"{t:.1f} s wall, {threads} threads"}
+                      f"{threads} threads; 1 warm-up + {repeats} timed runs, median "
+                      f"{q['median']:.2f} s (IQR {q['iqr']:.2f} s), engine wall clock"}
 
 
 def tune_summary(rep: dict) -> dict:
     best = rep["tried"][rep["best"]]
     return {"pilot_fraction": rep["pilot_fraction"], "chosen": best,
-            "tried": [[t["waves_per_group"], t["groups"], round(t["kernel_seconds"] * 1e3, 3)]
-                      for t in rep["tried"]]}
+            "confirm_fraction": rep["confirm_fraction"],
+            "tried": [[t["waves_per_group"], t["groups"], round(t["kernel_seconds"] * 1e3, 3),
+                       round(t["confirm_seconds"] * 1e3, 3)] for t in rep["tried"]]}
 
 
 def bench_batched(engine, steps: int, warmup: int, tune: bool) -> dict:

@@ -184,25 +184,31 @@ int tmc_hip_run_batched(const tmc_stack *stacks, int32_t n_stacks, const tmc_bat
 
 typedef struct tmc_tune_entry {
     int32_t waves_per_group, groups, lds_bytes, arz_tile_z, arz_tile_r;
-    double kernel_seconds; /* fastest pilot launch */
+    double kernel_seconds;  /* fastest pilot launch */
+    double confirm_seconds; /* faster of two launches at confirm_fraction; 0 if not re-timed */
 } tmc_tune_entry;
 
 typedef struct tmc_tune_report {
     double pilot_fraction;
-    int32_t n_tried, best; /* tried[best] is the shape chosen */
+    double confirm_fraction; /* 0 if no shape came close enough to the default to re-time */
+    int32_t n_tried, best;   /* tried[best] is the shape chosen */
     tmc_tune_entry tried[TMC_TUNE_MAX];
 } tmc_tune_report;
 
 /* Single runs: copies *opts into *tuned with waves_per_group, groups and (with TMC_FLAG_ARZ) the
  * A(r,z) tile set to the fastest candidate; the launch-shape fields of *opts are not used.
- * Candidates are timed in a fixed order, the default shape first, and one replaces the fastest
- * so far only if it is more than 3% faster (pilot timings scatter by about that much).
- * pilot_fraction is in (0, 1]. report may be NULL. */
+ * Every candidate is first timed on a pilot at pilot_fraction (fastest of three launches; the
+ * default shape is listed first). Short pilots are dominated by the last, longest photon walks,
+ * so their ranking is then confirmed: the default and up to two shapes whose pilot lies within
+ * 10% of the fastest are re-timed at confirm_fraction = min(1, 8 * pilot_fraction), and a shape
+ * replaces the default only if it is more than 3% faster there. pilot_fraction is in (0, 1].
+ * report may be NULL. */
 int tmc_autotune(const tmc_spec *spec, const tmc_opts *opts, double pilot_fraction,
                  tmc_opts *tuned, tmc_tune_report *report);
 
-/* Batched plans: times pilots of these stacks and rebuilds the plan's device state (and its
- * graph) with the fastest shape, which later executes use. report may be NULL. */
+/* Batched plans: times pilots of these stacks (with the same confirmation step) and rebuilds the
+ * plan's device state (and its graph) with the fastest shape, which later executes use. report
+ * may be NULL. */
 int tmc_batched_autotune(tmc_batched_plan *plan, const tmc_stack *stacks, int32_t n_stacks,
                          uint64_t seed, double pilot_fraction, tmc_tune_report *report);
 

@@ -21,22 +21,31 @@ def check_report(rep):
     tried = rep["tried"]
     assert len(tried) >= 3
     assert 0 <= rep["best"] < len(tried)
-    # in the order tried, a candidate replaces the best so far only if it is more than 3% faster;
-    # the choice is then within 3% of the fastest pilot
-    b = 0
-    for k in range(1, len(tried)):
-        if tried[k]["kernel_seconds"] < tried[b]["kernel_seconds"] * 0.97:
-            b = k
-    assert rep["best"] == b
-    assert tried[b]["kernel_seconds"] * 0.97 <= min(t["kernel_seconds"] for t in tried)
     assert all(t["kernel_seconds"] > 0 for t in tried)
     assert len({(t["waves_per_group"], t["groups"]) for t in tried}) == len(tried)
+    # contenders: up to two shapes besides the default (listed first) whose pilot is within 10%
+    # of the fastest pilot; they and the default are re-timed at 8x the pilot fraction, and a
+    # contender replaces the default only if it is more than 3% faster there
+    fastest = min(t["kernel_seconds"] for t in tried)
+    close = sorted((k for k in range(1, len(tried))
+                    if tried[k]["kernel_seconds"] <= fastest * 1.10),
+                   key=lambda k: tried[k]["kernel_seconds"])[:2]
+    confirmed = [k for k in range(len(tried)) if tried[k]["confirm_seconds"] > 0]
+    if not close:
+        assert rep["confirm_fraction"] == 0 and not confirmed and rep["best"] == 0
+        return
+    assert rep["confirm_fraction"] == min(1.0, 8 * rep["pilot_fraction"])
+    assert confirmed == sorted([0] + close)
+    t0 = tried[0]["confirm_seconds"]
+    wins = [k for k in close if tried[k]["confirm_seconds"] < t0 * 0.97]
+    want = min(wins, key=lambda k: tried[k]["confirm_seconds"]) if wins else 0
+    assert rep["best"] == want
 
 
 def test_single_run_tuned_shape_gives_identical_tallies(gpu):
     spec = load_spec("v3_three_layer")
     kw = dict(photons=400_000, batches=10, seed=5, arz=True)
-    t = engine.autotune(spec, pilot_fraction=0.25, **kw)
+    t = engine.autotune(spec, pilot_fraction=1 / 16, **kw)
     check_report(t["report"])
     best = t["report"]["tried"][t["report"]["best"]]
     assert t["run_kwargs"]["waves_per_group"] == best["waves_per_group"]

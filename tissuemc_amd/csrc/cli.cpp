@@ -174,9 +174,13 @@ int run_command(int argc, char **argv) {
             if (code != TMC_OK) throw std::runtime_error(tmc_last_error());
             const tmc_tune_entry &b = rep.tried[rep.best];
             std::printf("run %zu: autotune chose %d waves x %d workgroups (%d shapes, pilot %.3g, "
-                        "%.2f ms)\n",
+                        "%.2f ms",
                         r + 1, b.waves_per_group, b.groups, rep.n_tried, tune_fraction,
                         b.kernel_seconds * 1e3);
+            if (rep.confirm_fraction > 0)
+                std::printf("; confirmed at %.3g, %.2f ms", rep.confirm_fraction,
+                            b.confirm_seconds * 1e3);
+            std::printf(")\n");
         }
         const tmc::RunResult res = tmc::run_hip(runs[r], run_opt);
         const auto rd = res.rd(), tt = res.tt(), ab = res.absorbed();

@@ -1443,23 +1443,49 @@ uint64_t scaled_budget(uint64_t photons, double f, uint64_t at_least) {
     return std::max<uint64_t>(at_least, static_cast<uint64_t>(v));
 }
 
-// Times `pilot` at every candidate shape; returns the fastest tuning (base plus shape) and
-// fills the report. The tile of a single run follows from the LDS budget of the shape. A
-// candidate replaces the best so far only if its pilot is more than kTuneMargin faster: pilot
-// timings scatter by about that much, and a pilot that wins by less does not reliably win at the
-// full budget (the three-layer run: 12x1 beat 16x1 by 1.4% in a pilot and then ran 13% slower at
-// 1e8 photons). The candidates are listed with the default shape first.
+// Times a pilot at every candidate shape, then confirms the choice at a larger scale; returns
+// the fastest tuning (base plus shape) and fills the report. The tile of a single run follows
+// from the LDS budget of the shape.
+//
+// Pilot launches are short (about a dozen photons per lane at 1/32 of the three-layer run), so
+// their time is dominated by the last, longest photon walks and their ranking need not hold at
+// the full budget: on the three-layer run 12x1 beat 16x1 by 4% in the pilot and then ran 14%
+// slower at 1e8 photons. So every shape whose pilot lies within kContendFrac of the fastest pilot
+// is re-timed, with the default shape (listed first), at `confirm` (8 times the pilot fraction,
+// at most the full budget), and a shape replaces the default only if it is more than kTuneMargin
+// faster there.
+constexpr double kContendFrac = 0.10;
 constexpr double kTuneMargin = 0.03;
+constexpr int kMaxContenders = 3;
 
-tmc_tuning tune_shapes(const DeviceInfo &dev, const Problem &pilot, const tmc_tuning &base,
+struct Candidate {
+    tmc_tuning tuning;
+    int entry;  // index in the report
+};
+
+double time_shape(const DeviceInfo &dev, const Problem &p, const tmc_tuning &t, uint64_t seed,
+                  int launches, std::unique_ptr<Resident> *keep) {
+    auto res = std::make_unique<Resident>(dev, p, t);
+    res->upload(p, seed);
+    double tmin = 0.0;
+    for (int k = 0; k < launches; ++k) {
+        const double dt = res->run(false);
+        tmin = k == 0 ? dt : std::min(tmin, dt);
+    }
+    if (keep) *keep = std::move(res);
+    return tmin;
+}
+
+template <class MakeProblem>
+tmc_tuning tune_shapes(const DeviceInfo &dev, MakeProblem make_problem, const tmc_tuning &base,
                        uint64_t seed, double fraction, tmc_tune_report *rep) {
     tmc_tune_report local;
     tmc_tune_report &r = rep ? *rep : local;
     std::memset(&r, 0, sizeof r);
     r.pilot_fraction = fraction;
     r.best = -1;
-    tmc_tuning best = base;
-    double best_t = 0.0;
+    const Problem pilot = make_problem(fraction);
+    std::vector<Candidate> fits;
     for (const Shape &sh : kShapes) {
         if (r.n_tried >= TMC_TUNE_MAX) break;
         tmc_tuning t = base;
@@ -1468,18 +1494,17 @@ tmc_tuning tune_shapes(const DeviceInfo &dev, const Problem &pilot, const tmc_tu
         t.lds_budget = dev.lds_per_cu / static_cast<size_t>(sh.per_cu);
         t.arz_tile_z = t.arz_tile_r = -1;
         std::unique_ptr<Resident> res;
+        double tmin;
         try {
-            res = std::make_unique<Resident>(dev, pilot, t);
+            tmin = time_shape(dev, pilot, t, seed, 3, &res);
         } catch (const EngineError &e) {
             if (e.code() == TMC_ERR_CONFIG) continue;  // does not fit this shape's LDS budget
             throw;
         }
         if (res->waves_per_group() != sh.wpg) continue;  // the slots did not fit: shrunk
-        res->upload(pilot, seed);
-        double tmin = 0.0;
-        for (int k = 0; k < 3; ++k) {
-            const double dt = res->run(false);
-            tmin = k == 0 ? dt : std::min(tmin, dt);
+        if (pilot.arz) {
+            t.arz_tile_z = res->tile_z();
+            t.arz_tile_r = res->tile_r();
         }
         tmc_tune_entry &e = r.tried[r.n_tried];
         e.waves_per_group = sh.wpg;
@@ -1488,18 +1513,40 @@ tmc_tuning tune_shapes(const DeviceInfo &dev, const Problem &pilot, const tmc_tu
         e.arz_tile_z = res->tile_z();
         e.arz_tile_r = res->tile_r();
         e.kernel_seconds = tmin;
-        if (r.best < 0 || tmin < best_t * (1.0 - kTuneMargin)) {
-            r.best = r.n_tried;
-            best_t = tmin;
-            best = t;
-            if (pilot.arz) {
-                best.arz_tile_z = res->tile_z();
-                best.arz_tile_r = res->tile_r();
-            }
-        }
+        fits.push_back({t, r.n_tried});
         ++r.n_tried;
     }
-    if (r.best < 0) throw EngineError(TMC_ERR_CONFIG, "no launch shape fits this problem's LDS tallies");
+    if (fits.empty()) throw EngineError(TMC_ERR_CONFIG, "no launch shape fits this problem's LDS tallies");
+
+    // Contenders: the default (first fitting shape) and the shapes whose pilot is close to the
+    // fastest pilot, fastest first.
+    double fastest = r.tried[fits[0].entry].kernel_seconds;
+    for (const Candidate &c : fits) fastest = std::min(fastest, r.tried[c.entry].kernel_seconds);
+    std::vector<Candidate> others;
+    for (size_t i = 1; i < fits.size(); ++i)
+        if (r.tried[fits[i].entry].kernel_seconds <= fastest * (1.0 + kContendFrac)) others.push_back(fits[i]);
+    std::sort(others.begin(), others.end(), [&](const Candidate &a, const Candidate &b) {
+        return r.tried[a.entry].kernel_seconds < r.tried[b.entry].kernel_seconds;
+    });
+    if (others.size() > kMaxContenders - 1) others.resize(kMaxContenders - 1);
+    r.best = fits[0].entry;
+    tmc_tuning best = fits[0].tuning;
+    if (others.empty()) return best;
+
+    r.confirm_fraction = std::min(1.0, 8.0 * fraction);
+    const Problem big = make_problem(r.confirm_fraction);
+    const double t_default = time_shape(dev, big, fits[0].tuning, seed, 2, nullptr);
+    r.tried[fits[0].entry].confirm_seconds = t_default;
+    double best_t = t_default;
+    for (const Candidate &c : others) {
+        const double t = time_shape(dev, big, c.tuning, seed, 2, nullptr);
+        r.tried[c.entry].confirm_seconds = t;
+        if (t < t_default * (1.0 - kTuneMargin) && t < best_t) {
+            best_t = t;
+            best = c.tuning;
+            r.best = c.entry;
+        }
+    }
     return best;
 }
 
@@ -1510,12 +1557,14 @@ void autotune_single(const tmc_spec &spec_in, const tmc_opts &opt, double pilot_
     const double f = check_fraction(pilot_fraction);
     const tmc_spec spec = checked_single(spec_in, opt);
     const DeviceInfo dev = device_info(opt.device);
-    tmc_spec ps = spec;
-    ps.photons = scaled_budget(spec.photons, f, static_cast<uint64_t>(opt.batches));
     tmc_opts po = opt;
     po.photons = 0;
-    const Problem pilot = single_problem(ps, po);
-    const tmc_tuning best = tune_shapes(dev, pilot, tuning_of(opt), opt.seed, f, report);
+    auto make_problem = [&](double fraction) {
+        tmc_spec ps = spec;
+        ps.photons = scaled_budget(spec.photons, fraction, static_cast<uint64_t>(opt.batches));
+        return single_problem(ps, po);
+    };
+    const tmc_tuning best = tune_shapes(dev, make_problem, tuning_of(opt), opt.seed, f, report);
     *tuned = opt;
     tuned->waves_per_group = best.waves_per_group;
     tuned->groups = best.groups;
@@ -1530,22 +1579,26 @@ void batched_autotune(BatchedPlan &plan, const tmc_stack *stacks, int n_stacks, 
         throw EngineError(TMC_ERR_CONFIG, "the plan was made for " + std::to_string(plan.shape.n_prows) +
                                               " stacks, got " + std::to_string(n_stacks));
     const uint64_t Bs = static_cast<uint64_t>(plan.opt.batches_per_stack);
-    std::vector<tmc_stack> ps(stacks, stacks + n_stacks);
     for (int s = 0; s < n_stacks; ++s) {
         check_stack(stacks[s], s);
         if (stacks[s].photons < Bs)
             throw EngineError(TMC_ERR_CONFIG, "stack " + std::to_string(s) +
                                                   ": batches cannot exceed the number of photons");
-        ps[static_cast<size_t>(s)].photons = scaled_budget(stacks[s].photons, f, Bs);
     }
-    Problem pilot = plan.shape;
-    batched_units(ps.data(), n_stacks, plan.opt.batches_per_stack, pilot);
-    batched_stacks(ps.data(), n_stacks, plan.opt, pilot);
+    auto make_problem = [&](double fraction) {
+        std::vector<tmc_stack> ps(stacks, stacks + n_stacks);
+        for (int s = 0; s < n_stacks; ++s)
+            ps[static_cast<size_t>(s)].photons = scaled_budget(stacks[s].photons, fraction, Bs);
+        Problem pilot = plan.shape;
+        batched_units(ps.data(), n_stacks, plan.opt.batches_per_stack, pilot);
+        batched_stacks(ps.data(), n_stacks, plan.opt, pilot);
+        return pilot;
+    };
     tmc_tuning base;
     base.chunk = plan.opt.chunk;
     base.step_limit = plan.opt.step_limit;
     base.arz_tile_z = base.arz_tile_r = 0;
-    const tmc_tuning best = tune_shapes(plan.dev, pilot, base, seed, f, report);
+    const tmc_tuning best = tune_shapes(plan.dev, make_problem, base, seed, f, report);
     plan.res.reset();
     plan.res = std::make_unique<Resident>(plan.dev, plan.shape, best);
     plan.opt.waves_per_group = best.waves_per_group;

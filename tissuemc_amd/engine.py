@@ -97,11 +97,13 @@ TUNE_MAX = 16
 
 class TmcTuneEntry(C.Structure):
     _fields_ = [("waves_per_group", C.c_int32), ("groups", C.c_int32), ("lds_bytes", C.c_int32),
-                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("kernel_seconds", C.c_double)]
+                ("arz_tile_z", C.c_int32), ("arz_tile_r", C.c_int32), ("kernel_seconds", C.c_double),
+                ("confirm_seconds", C.c_double)]
 
 
 class TmcTuneReport(C.Structure):
-    _fields_ = [("pilot_fraction", C.c_double), ("n_tried", C.c_int32), ("best", C.c_int32),
+    _fields_ = [("pilot_fraction", C.c_double), ("confirm_fraction", C.c_double),
+                ("n_tried", C.c_int32), ("best", C.c_int32),
                 ("tried", TmcTuneEntry * TUNE_MAX)]
 
 
@@ -262,9 +264,11 @@ def _opts(*, seed, batches, device, photons, stream, arz, profiles, waves_per_gr
 
 def _report(r: TmcTuneReport) -> dict:
     tried = [{"waves_per_group": e.waves_per_group, "groups": e.groups, "lds_bytes": e.lds_bytes,
-              "arz_tile": [e.arz_tile_z, e.arz_tile_r], "kernel_seconds": e.kernel_seconds}
+              "arz_tile": [e.arz_tile_z, e.arz_tile_r], "kernel_seconds": e.kernel_seconds,
+              "confirm_seconds": e.confirm_seconds}
              for e in r.tried[:r.n_tried]]
-    return {"pilot_fraction": r.pilot_fraction, "best": r.best, "tried": tried}
+    return {"pilot_fraction": r.pilot_fraction, "confirm_fraction": r.confirm_fraction,
+            "best": r.best, "tried": tried}
 
 
 def autotune(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, device: int = 0,
