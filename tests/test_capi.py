@@ -252,3 +252,32 @@ def test_error_messages_match_the_reference_parser(tmp_path, mutation):
     assert str(err.value) == want
     ours = cli("run", str(path))
     assert ours.returncode == 1 and ours.stderr.strip() == ref.stderr.strip()
+
+
+@pytest.mark.parametrize("text", [
+    THREE_LAYER,
+    THREE_LAYER.replace("\n", "\r\n"),
+    THREE_LAYER.replace(" ", "\t").rstrip("\n"),
+    "1.0\n1\nx.mco A\n+50\n1E-2 .5e-1\n7 9 3\n1\n+1.0\n1.33 0x1p-2 1e2 -0.5 3.5 # hex mua\n1.2",
+])
+def test_parsed_values_match_the_reference_parser(tmp_path, text):
+    """Valid inputs in unusual spellings (CRLF, tabs, signs, exponents, hex, no final newline)
+    read to the same run as the reference engine reads, per the input echo of its JSON."""
+    import json
+
+    import oracle
+
+    if not oracle.ref_available():
+        pytest.skip("the reference engine is not built here")
+    path = tmp_path / "input.mci"
+    path.write_bytes(text.encode())
+    out = tmp_path / "ref.json"
+    subprocess.run([str(oracle.REF_BIN), "run", str(path), "--photons", "2", "--batches", "1",
+                    "--output", str(out)], check=True, capture_output=True)
+    ref = json.loads(out.read_text())["runs"][0]
+    (ours,) = engine.parse_mci(path=path)
+    assert ours["output_name"] == ref["output_name"]
+    assert ours["layers"] == ref["layers"]
+    assert (ours["dz"], ours["dr"], ours["nz"], ours["nr"]) == tuple(
+        ref["grid"][k] for k in ("dz", "dr", "nz", "nr"))
+    assert (ours["n_above"], ours["n_below"]) == (ref["n_above"], ref["n_below"])

@@ -1,9 +1,9 @@
 // The C ABI (include/tissuemc_hip.h): error mapping, run validation and the .mci reader.
 //
-// The .mci reader accepts the MCML input format the reference reads (src/io/mci.cpp:97-171):
-// '#' starts a comment, blank lines are skipped, values come in a fixed order, counts must be
-// integers, and every error names its line. Validation rules and messages follow
+// The .mci reader accepts the MCML input format the reference reads (src/io/mci.cpp:97-171),
+// with the same messages; run validation rules and messages follow
 // include/tissuemc/config.hpp:38-66.
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -12,6 +12,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "engine.h"
@@ -42,130 +43,167 @@ template <class F> int guarded(F &&f) {
 [[noreturn]] void config_fail(const std::string &msg) { throw tmc::EngineError(TMC_ERR_CONFIG, msg); }
 
 // ------------------------------------------------------------------ .mci reader
+//
+// An .mci file is a fixed sequence of records, one per content line ('#' starts a comment;
+// lines with nothing else are skipped):
+//   version | run count | per run: output name + format, photons, dz dr, nz nr na, layer count,
+//   n above, one line per layer (n mua mus g d), n below.
+// The reader walks the text once with a cursor that yields content lines as views into the text,
+// and a record declares how many fields it has and what it is called in messages. Field values
+// are converted by kind: reals must be finite, counts must be base-10 integers (MCML's manual
+// asks for "5", not "5.0"). Every message names the line it is about; the wording is the
+// reference parser's (src/io/mci.cpp), which users' scripts may match.
 
-struct McLine {
-    int number;
-    std::vector<std::string> words;
-};
-
-[[noreturn]] void line_fail(int line, const std::string &msg) {
+[[noreturn]] void fail_at(int line, const std::string &msg) {
     config_fail("line " + std::to_string(line) + ": " + msg);
 }
 
-class McReader {
-  public:
-    explicit McReader(const std::string &text) {
-        std::istringstream in(text);
-        std::string raw;
-        int number = 0;
-        while (std::getline(in, raw)) {
-            ++number;
-            const size_t hash = raw.find('#');
-            if (hash != std::string::npos) raw.resize(hash);
-            std::istringstream ws(raw);
-            McLine l{number, {}};
-            for (std::string w; ws >> w;) l.words.push_back(w);
-            if (!l.words.empty()) lines_.push_back(std::move(l));
-        }
+struct Record {
+    int line = 0;
+    std::vector<std::string_view> fields;
+
+    std::string text(size_t i) const { return std::string(fields[i]); }
+
+    double real(size_t i, const std::string &name) const {
+        const std::string t = text(i);
+        char *end = nullptr;
+        errno = 0;
+        const double v = std::strtod(t.c_str(), &end);
+        const bool whole = end != t.c_str() && *end == '\0';
+        if (!whole || errno == ERANGE || !std::isfinite(v))
+            fail_at(line, name + ": '" + t + "' is not a finite number");
+        return v;
     }
 
-    const McLine &take(size_t count, const char *what) {
-        if (pos_ >= lines_.size()) {
-            const int last = lines_.empty() ? 0 : lines_.back().number;
-            line_fail(last, std::string("unexpected end of file, expected ") + what);
-        }
-        const McLine &l = lines_[pos_++];
-        if (l.words.size() != count)
-            line_fail(l.number, std::string(what) + ": expected " + std::to_string(count) +
-                                    " value(s), found " + std::to_string(l.words.size()));
-        return l;
+    long long count(size_t i, const std::string &name) const {
+        const std::string t = text(i);
+        char *end = nullptr;
+        errno = 0;
+        const long long v = std::strtoll(t.c_str(), &end, 10);
+        if (end == t.c_str() || *end != '\0' || errno == ERANGE)
+            fail_at(line, name + ": '" + t + "' is not an integer");
+        return v;
     }
-
-    bool done() const { return pos_ >= lines_.size(); }
-    int next_line() const { return lines_[pos_].number; }
-
-  private:
-    std::vector<McLine> lines_;
-    size_t pos_ = 0;
 };
 
-double real_at(const McLine &l, size_t i, const std::string &what) {
-    const char *s = l.words[i].c_str();
-    char *end = nullptr;
-    errno = 0;
-    const double v = std::strtod(s, &end);
-    if (end == s || *end != '\0' || errno == ERANGE || !std::isfinite(v))
-        line_fail(l.number, what + ": '" + l.words[i] + "' is not a finite number");
-    return v;
-}
+class RecordCursor {
+  public:
+    explicit RecordCursor(std::string_view text) : rest_(text) { advance(); }
 
-long long int_at(const McLine &l, size_t i, const std::string &what) {
-    const char *s = l.words[i].c_str();
-    char *end = nullptr;
-    errno = 0;
-    const long long v = std::strtoll(s, &end, 10);
-    if (end == s || *end != '\0' || errno == ERANGE)
-        line_fail(l.number, what + ": '" + l.words[i] + "' is not an integer");
-    return v;
-}
-
-std::vector<tmc_spec> parse_text(const std::string &text) {
-    McReader rd(text);
-    const McLine &ver = rd.take(1, "file version");
-    if (real_at(ver, 0, "file version") != 1.0)
-        line_fail(ver.number, "unsupported file version '" + ver.words[0] + "' (expected 1.0)");
-    const McLine &nrl = rd.take(1, "number of runs");
-    const long long nruns = int_at(nrl, 0, "number of runs");
-    if (nruns < 1) line_fail(nrl.number, "number of runs must be >= 1");
-    std::vector<tmc_spec> runs;
-    for (long long r = 0; r < nruns; ++r) {
-        tmc_spec s;
-        std::memset(&s, 0, sizeof s);
-        const McLine &o = rd.take(2, "output filename and format");
-        std::snprintf(s.output_name, sizeof s.output_name, "%s", o.words[0].c_str());
-        if (o.words[1] != "A")
-            line_fail(o.number, "output format must be 'A' (ASCII), found '" + o.words[1] + "'");
-        const McLine &np = rd.take(1, "number of photons");
-        const long long photons = int_at(np, 0, "number of photons");
-        if (photons < 1) line_fail(np.number, "number of photons must be >= 1");
-        s.photons = static_cast<uint64_t>(photons);
-        const McLine &st = rd.take(2, "dz and dr");
-        s.dz = real_at(st, 0, "dz");
-        s.dr = real_at(st, 1, "dr");
-        const McLine &gr = rd.take(3, "grid counts (nz nr na)");
-        const long long nz = int_at(gr, 0, "nz"), nr = int_at(gr, 1, "nr"), na = int_at(gr, 2, "na");
-        constexpr long long kMaxBins = 1000000;
-        if (nz < 1 || nr < 1 || na < 1 || nz > kMaxBins || nr > kMaxBins || na > kMaxBins)
-            line_fail(gr.number, "grid counts must be integers in [1, 1000000]");
-        s.nz = static_cast<int32_t>(nz);
-        s.nr = static_cast<int32_t>(nr);
-        s.na = static_cast<int32_t>(na);
-        const McLine &nl = rd.take(1, "number of layers");
-        const long long layers = int_at(nl, 0, "number of layers");
-        if (layers < 1 || layers > tmc::kMaxLayers)
-            line_fail(nl.number, "number of layers must be in [1, " + std::to_string(tmc::kMaxLayers) + "]");
-        s.num_layers = static_cast<int32_t>(layers);
-        s.n_above = real_at(rd.take(1, "refractive index above"), 0, "n above");
-        for (long long i = 0; i < layers; ++i) {
-            const McLine &ll = rd.take(5, "layer (n mua mus g d)");
-            const std::string tag = "layer " + std::to_string(i + 1);
-            tmc_layer &L = s.layers[i];
-            L.n = real_at(ll, 0, tag + " n");
-            L.mua = real_at(ll, 1, tag + " mua");
-            L.mus = real_at(ll, 2, tag + " mus");
-            L.g = real_at(ll, 3, tag + " g");
-            L.d = real_at(ll, 4, tag + " thickness");
-        }
-        const McLine &below = rd.take(1, "refractive index below");
-        s.n_below = real_at(below, 0, "n below");
-        try {
-            tmc::validate_spec(s);
-        } catch (const tmc::EngineError &e) {
-            line_fail(below.number, std::string("invalid run: ") + e.what());
-        }
-        runs.push_back(s);
+    // The next record, which must hold exactly `fields` values.
+    Record expect(size_t fields, const char *what) {
+        if (!has_next_)
+            fail_at(last_line_, std::string("unexpected end of file, expected ") + what);
+        Record r = std::move(next_);
+        advance();
+        if (r.fields.size() != fields)
+            fail_at(r.line, std::string(what) + ": expected " + std::to_string(fields) +
+                                " value(s), found " + std::to_string(r.fields.size()));
+        return r;
     }
-    if (!rd.done()) line_fail(rd.next_line(), "unexpected extra content after the last run");
+
+    bool at_end() const { return !has_next_; }
+    int next_line() const { return next_.line; }
+
+  private:
+    // Moves to the next line with content, splitting it into whitespace-separated fields.
+    void advance() {
+        has_next_ = false;
+        while (!rest_.empty() && !has_next_) {
+            const size_t nl = rest_.find('\n');
+            std::string_view line = rest_.substr(0, nl);
+            rest_ = nl == std::string_view::npos ? std::string_view() : rest_.substr(nl + 1);
+            ++line_no_;
+            line = line.substr(0, line.find('#'));
+            Record r;
+            r.line = line_no_;
+            size_t i = 0;
+            while (i < line.size()) {
+                while (i < line.size() && std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+                size_t j = i;
+                while (j < line.size() && !std::isspace(static_cast<unsigned char>(line[j]))) ++j;
+                if (j > i) r.fields.push_back(line.substr(i, j - i));
+                i = j;
+            }
+            if (!r.fields.empty()) {
+                next_ = std::move(r);
+                has_next_ = true;
+                last_line_ = line_no_;
+            }
+        }
+    }
+
+    std::string_view rest_;
+    int line_no_ = 0, last_line_ = 0;
+    Record next_;
+    bool has_next_ = false;
+};
+
+constexpr long long kMaxBins = 1000000;
+
+tmc_spec read_run(RecordCursor &in) {
+    tmc_spec s;
+    std::memset(&s, 0, sizeof s);
+    const Record out = in.expect(2, "output filename and format");
+    std::snprintf(s.output_name, sizeof s.output_name, "%s", out.text(0).c_str());
+    if (out.fields[1] != "A")
+        fail_at(out.line, "output format must be 'A' (ASCII), found '" + out.text(1) + "'");
+
+    const Record np = in.expect(1, "number of photons");
+    const long long photons = np.count(0, "number of photons");
+    if (photons < 1) fail_at(np.line, "number of photons must be >= 1");
+    s.photons = static_cast<uint64_t>(photons);
+
+    const Record widths = in.expect(2, "dz and dr");
+    s.dz = widths.real(0, "dz");
+    s.dr = widths.real(1, "dr");
+
+    const Record grid = in.expect(3, "grid counts (nz nr na)");
+    long long counts[3];
+    const char *names[3] = {"nz", "nr", "na"};
+    for (size_t k = 0; k < 3; ++k) counts[k] = grid.count(k, names[k]);
+    for (long long c : counts)
+        if (c < 1 || c > kMaxBins) fail_at(grid.line, "grid counts must be integers in [1, 1000000]");
+    s.nz = static_cast<int32_t>(counts[0]);
+    s.nr = static_cast<int32_t>(counts[1]);
+    s.na = static_cast<int32_t>(counts[2]);
+
+    const Record nl = in.expect(1, "number of layers");
+    const long long layers = nl.count(0, "number of layers");
+    if (layers < 1 || layers > tmc::kMaxLayers)
+        fail_at(nl.line, "number of layers must be in [1, " + std::to_string(tmc::kMaxLayers) + "]");
+    s.num_layers = static_cast<int32_t>(layers);
+
+    s.n_above = in.expect(1, "refractive index above").real(0, "n above");
+    static const char *const kLayerFields[5] = {" n", " mua", " mus", " g", " thickness"};
+    for (int i = 0; i < s.num_layers; ++i) {
+        const Record L = in.expect(5, "layer (n mua mus g d)");
+        const std::string tag = "layer " + std::to_string(i + 1);
+        double v[5];
+        for (size_t k = 0; k < 5; ++k) v[k] = L.real(k, tag + kLayerFields[k]);
+        s.layers[i] = tmc_layer{v[0], v[1], v[2], v[3], v[4]};
+    }
+    const Record below = in.expect(1, "refractive index below");
+    s.n_below = below.real(0, "n below");
+    try {
+        tmc::validate_spec(s);
+    } catch (const tmc::EngineError &e) {
+        fail_at(below.line, std::string("invalid run: ") + e.what());
+    }
+    return s;
+}
+
+std::vector<tmc_spec> parse_text(std::string_view text) {
+    RecordCursor in(text);
+    const Record version = in.expect(1, "file version");
+    if (version.real(0, "file version") != 1.0)
+        fail_at(version.line, "unsupported file version '" + version.text(0) + "' (expected 1.0)");
+    const Record nr = in.expect(1, "number of runs");
+    const long long n = nr.count(0, "number of runs");
+    if (n < 1) fail_at(nr.line, "number of runs must be >= 1");
+    std::vector<tmc_spec> runs;
+    for (long long r = 0; r < n; ++r) runs.push_back(read_run(in));
+    if (!in.at_end()) fail_at(in.next_line(), "unexpected extra content after the last run");
     return runs;
 }
 
