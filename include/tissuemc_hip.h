@@ -290,6 +290,21 @@ int tmc_hip_available(void);
  * the reference's CPU engine. */
 int tmc_hip_run(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out);
 
+/* tmc_hip_run for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Same
+ * arguments, results and errors as tmc_hip_run, bit for bit, but it calls no hipMalloc or
+ * hipFree and creates no stream or event: every device buffer of the run (unit and layer tables,
+ * seed, tallies) comes from one static per-device scratch array of tmc_static_scratch_bytes()
+ * (16 MiB; every GPU the library runs on holds it until the process ends), and the work goes to
+ * the null stream. A run needing more scratch (about 8 bytes per tally bin and per-batch
+ * counter) fails with TMC_ERR_CONFIG before any device work. kernel_seconds is measured on the
+ * host from the transport launch to its completion. All calls on one GPU share its scratch
+ * array: calls that overlap on one GPU (from several host threads) race and must not be made. */
+int tmc_hip_run_static(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out);
+
+/* Bytes of the static per-device scratch array used by tmc_hip_run_static. */
+uint64_t tmc_static_scratch_bytes(void);
+
 /* Message for the last failed call on this thread ("" if none). */
 const char *tmc_last_error(void);
 

@@ -31,7 +31,7 @@ comp.mco A # output filename, ASCII/Binary
 
 def declared_functions() -> list[str]:
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|void|const char \*)\s*(tmc_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|uint64_t|double|void|const char \*)\s*(tmc_\w+)\s*\(",
                                  text, flags=re.M)))
 
 
@@ -281,3 +281,17 @@ def test_parsed_values_match_the_reference_parser(tmp_path, text):
     assert (ours["dz"], ours["dr"], ours["nz"], ours["nr"]) == tuple(
         ref["grid"][k] for k in ("dz", "dr", "nz", "nr"))
     assert (ours["n_above"], ours["n_below"]) == (ref["n_above"], ref["n_below"])
+
+
+def test_static_run_rejects_a_run_beyond_its_scratch_before_any_device_work():
+    """tmc_hip_run_static takes every device buffer from a fixed 16 MiB array: a run whose
+    tallies need more (here A(r,z) on 1500 x 1500 bins, 18 MB) is refused with a clear error
+    before any device call, on any machine."""
+    L = engine.load()
+    assert L.tmc_static_scratch_bytes() == 16 << 20
+    (spec,) = engine.parse_mci(THREE_LAYER)
+    big = dict(spec, nz=1500, nr=1500)
+    with pytest.raises(engine.ConfigError, match="bytes of device scratch; the static buffer holds 16777216"):
+        engine.run(big, photons=100, batches=2, arz=True, no_alloc=True)
+    with pytest.raises(engine.ConfigError, match="batches must be >= 1"):
+        engine.run(spec, photons=100, batches=0, no_alloc=True)

@@ -274,6 +274,17 @@ int tmc_hip_run(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out) {
     });
 }
 
+int tmc_hip_run_static(const tmc_spec *spec, const tmc_opts *opts, tmc_out *out) {
+    return guarded([&] {
+        if (!spec || !opts || !out) config_fail("spec, opts and out must not be NULL");
+        if (!out->rd || !out->tt || !out->absorbed || !out->events || !out->anomalies)
+            config_fail("tmc_out: rd, tt, absorbed, events and anomalies are required");
+        tmc::run_transport_static(*spec, *opts, out);
+    });
+}
+
+uint64_t tmc_static_scratch_bytes(void) { return tmc::static_scratch_bytes(); }
+
 int tmc_parse_mci_text(const char *text, tmc_spec *runs, int32_t max_runs, int32_t *n_runs) {
     return guarded([&] {
         if (!text || !runs) config_fail("text and runs must not be NULL");

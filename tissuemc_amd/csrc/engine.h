@@ -27,6 +27,10 @@ void validate_spec(const tmc_spec &spec);
 
 // Runs the transport kernel for one run and fills `out` (capi.cpp checks pointers first).
 void run_transport(const tmc_spec &spec, const tmc_opts &opts, tmc_out *out);
+// The same with every device buffer taken from a static per-device scratch array of
+// static_scratch_bytes(): no device allocation at all.
+void run_transport_static(const tmc_spec &spec, const tmc_opts &opts, tmc_out *out);
+size_t static_scratch_bytes();
 
 // Launch-shape choices shared by single and batched runs (0 / -1 = automatic).
 struct tmc_tuning {

@@ -47,6 +47,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -892,11 +893,35 @@ struct Problem {
     int reach_z = 1;  // depth bins that can receive weight (for the A(r,z) tile)
 };
 
+// A fixed region of device memory that a Resident carves its buffers from instead of allocating
+// them (the no-allocation entry point): 256-byte aligned pieces, in order, from `base`.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+
+// Device bytes a Resident for problem `p` takes from an Arena: the unit table, the stack table,
+// the key and the cleared output range, each rounded up to 256 bytes.
+size_t arena_bytes(const Problem &p) {
+    auto up = [](size_t b) { return (std::max<size_t>(b, 8) + 255) / 256 * 256; };
+    const size_t U = p.units.size(), S = p.n_srows, PR = p.n_prows;
+    const size_t out_words = 1 + U + 5 * S + S * p.abs_stride +
+                             (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
+                             (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
+                             (p.na > 0 ? 2 * static_cast<size_t>(p.na) * (1 + p.nr) : 0) +
+                             2 * static_cast<size_t>(p.nt);
+    return up(U * sizeof(UnitRec)) + up(p.stacks.size() * sizeof(float)) + up(8) + up(out_words * 8);
+}
+
 // Device memory, kernel arguments and launch shape for one problem shape. A batched plan keeps
-// one of these and replays a captured graph (clear + transport) on it.
+// one of these and replays a captured graph (clear + transport) on it. With an Arena, buffers
+// come from the arena instead of hipMalloc, work goes to the null stream and no events are
+// created (the kernel is timed on the host around its completion).
 class Resident {
   public:
-    Resident(const DeviceInfo &dev, const Problem &p, const tmc_tuning &t) : dev_(dev) {
+    Resident(const DeviceInfo &dev, const Problem &p, const tmc_tuning &t, Arena *arena = nullptr)
+        : dev_(dev), arena_(arena) {
         std::memset(&a_, 0, sizeof a_);
         a_.n_units = static_cast<uint32_t>(p.units.size());
         a_.stack_stride = p.stack_stride;
@@ -1017,9 +1042,11 @@ class Resident {
         n_srows_ = S;
         n_prows_ = PR;
 
-        HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreate(&e0_));
-        HIP_TRY(hipEventCreate(&e1_));
+        if (!arena_) {
+            HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreate(&e0_));
+            HIP_TRY(hipEventCreate(&e1_));
+        }
         const void *k = kernel();
         HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(lds_bytes_)));
@@ -1061,6 +1088,14 @@ class Resident {
             HIP_TRY(hipEventRecord(e0_, stream_));
             HIP_TRY(hipGraphLaunch(graph_exec_, stream_));
             HIP_TRY(hipEventRecord(e1_, stream_));
+        } else if (!e0_) {
+            // no events (arena mode): host clock from the launch to its completion
+            enqueue_clear();
+            HIP_TRY(hipStreamSynchronize(stream_));
+            const auto t0 = std::chrono::steady_clock::now();
+            enqueue_transport();
+            HIP_TRY(hipStreamSynchronize(stream_));
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         } else {
             enqueue_clear();
             HIP_TRY(hipEventRecord(e0_, stream_));
@@ -1122,6 +1157,14 @@ class Resident {
 
   private:
     void *alloc(size_t bytes) {
+        if (arena_) {
+            const size_t need = (std::max<size_t>(bytes, 8) + 255) / 256 * 256;
+            if (arena_->used + need > arena_->size)
+                throw EngineError(TMC_ERR_CONFIG, "the static device scratch is too small for this run");
+            void *p = arena_->base + arena_->used;
+            arena_->used += need;
+            return p;
+        }
         void *p = nullptr;
         HIP_TRY(hipMalloc(&p, std::max<size_t>(bytes, 8)));
         allocs_.push_back(p);
@@ -1149,6 +1192,7 @@ class Resident {
     }
 
     DeviceInfo dev_;
+    Arena *arena_ = nullptr;
     KernelArgs a_;
     bool profiles_ = true, arz_ = false, agg_ = false, exit_ = false;
     int wpg_ = 16, groups_ = 0;
@@ -1249,22 +1293,23 @@ tmc_tuning tuning_of(const tmc_opts &opt) {
 
 }  // namespace
 
-void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
-    const auto t0 = std::chrono::steady_clock::now();
-    const tmc_spec spec = checked_single(spec_in, opt);
-    const bool want_arz = (opt.flags & TMC_FLAG_ARZ) != 0;
-    if (want_arz && out->arz == nullptr)
-        throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
-    const bool want_angles = (opt.flags & TMC_FLAG_ANGLES) != 0;
-    if (want_angles && (!out->ra || !out->ta || !out->rra || !out->tra))
-        throw EngineError(TMC_ERR_CONFIG, "angle profiles requested but tmc_out.ra/ta/rra/tra is NULL");
-    const bool want_time = (opt.flags & TMC_FLAG_TIME) != 0;
-    if (want_time && (!out->rt || !out->tt_t))
-        throw EngineError(TMC_ERR_CONFIG, "time-resolved profiles requested but tmc_out.rt/tt_t is NULL");
-    const DeviceInfo dev = device_info(opt.device);
+namespace {
 
-    const Problem p = single_problem(spec, opt);
-    Resident r(dev, p, tuning_of(opt));
+void check_out(const tmc_opts &opt, const tmc_out *out) {
+    if ((opt.flags & TMC_FLAG_ARZ) != 0 && out->arz == nullptr)
+        throw EngineError(TMC_ERR_CONFIG, "A(r,z) requested but tmc_out.arz is NULL");
+    if ((opt.flags & TMC_FLAG_ANGLES) != 0 && (!out->ra || !out->ta || !out->rra || !out->tra))
+        throw EngineError(TMC_ERR_CONFIG, "angle profiles requested but tmc_out.ra/ta/rra/tra is NULL");
+    if ((opt.flags & TMC_FLAG_TIME) != 0 && (!out->rt || !out->tt_t))
+        throw EngineError(TMC_ERR_CONFIG, "time-resolved profiles requested but tmc_out.rt/tt_t is NULL");
+}
+
+// Upload, run and download one single run on `r`; fills every field of `out`.
+void run_single_on(Resident &r, const DeviceInfo &dev, const Problem &p, const tmc_spec &spec,
+                   const tmc_opts &opt, tmc_out *out) {
+    const bool want_arz = (opt.flags & TMC_FLAG_ARZ) != 0;
+    const bool want_angles = (opt.flags & TMC_FLAG_ANGLES) != 0;
+    const bool want_time = (opt.flags & TMC_FLAG_TIME) != 0;
     r.upload(p, opt.seed);
     out->kernel_seconds = r.run(false);
     r.download(out->rd, out->tt, out->absorbed, out->events, out->anomalies, out->boundary_steps,
@@ -1284,6 +1329,67 @@ void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
     out->chunk = static_cast<int32_t>(r.chunk());
     out->arz_tile_z = r.tile_z();
     out->arz_tile_r = r.tile_r();
+}
+
+// ---- the no-allocation entry point's device scratch
+//
+// kScratchBytes of device memory per GPU, part of this library's code object: every GPU the
+// library runs on holds it from the library's first use there until the process ends. In host code `scratch` names a host-side stand-in, not device memory:
+// its device address is taken once per device with hipGetSymbolAddress, with that device
+// current, and only that address is used.
+constexpr size_t kScratchWords = size_t{2} << 20;
+constexpr size_t kScratchBytes = kScratchWords * 8;  // 16 MiB
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch_ptr;  // per device ordinal; null until set up
+
+// The device address of `scratch` on the current device `device`.
+void *static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr.size() <= static_cast<size_t>(device)) g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+    void *&ptr = g_scratch_ptr[static_cast<size_t>(device)];
+    if (!ptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            throw EngineError(TMC_ERR_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr = p;
+    }
+    return ptr;
+}
+
+}  // namespace
+
+void run_transport(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const tmc_spec spec = checked_single(spec_in, opt);
+    check_out(opt, out);
+    const DeviceInfo dev = device_info(opt.device);
+    const Problem p = single_problem(spec, opt);
+    Resident r(dev, p, tuning_of(opt));
+    run_single_on(r, dev, p, spec, opt, out);
+    out->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+size_t static_scratch_bytes() { return kScratchBytes; }
+
+void run_transport_static(const tmc_spec &spec_in, const tmc_opts &opt, tmc_out *out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const tmc_spec spec = checked_single(spec_in, opt);
+    check_out(opt, out);
+    const Problem p = single_problem(spec, opt);
+    const size_t need = arena_bytes(p);
+    if (need > kScratchBytes)
+        throw EngineError(TMC_ERR_CONFIG, "this run needs " + std::to_string(need) +
+                                              " bytes of device scratch; the static buffer holds " +
+                                              std::to_string(kScratchBytes) + " (use tmc_hip_run)");
+    const DeviceInfo dev = device_info(opt.device);
+    Arena arena;
+    arena.base = static_cast<char *>(static_scratch(dev.id));
+    arena.size = kScratchBytes;
+    Resident r(dev, p, tuning_of(opt), &arena);
+    run_single_on(r, dev, p, spec, opt, out);
     out->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

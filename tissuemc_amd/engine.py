@@ -114,7 +114,8 @@ EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_poi
             "tmc_generate_stacks", "tmc_mlp_default_config", "tmc_mlp_create", "tmc_mlp_destroy",
             "tmc_mlp_param_count", "tmc_mlp_set_data", "tmc_mlp_train", "tmc_mlp_gradients",
             "tmc_mlp_predict", "tmc_mlp_get_params", "tmc_mlp_set_params", "tmc_mlp_set_lr",
-            "tmc_mlp_flops", "tmc_autotune", "tmc_batched_autotune")
+            "tmc_mlp_flops", "tmc_autotune", "tmc_batched_autotune", "tmc_hip_run_static",
+            "tmc_static_scratch_bytes")
 
 _lib: C.CDLL | None = None
 
@@ -131,6 +132,9 @@ def load() -> C.CDLL:
     L.tmc_hip_available.restype = C.c_int
     L.tmc_hip_run.argtypes = [C.POINTER(TmcSpec), C.POINTER(TmcOpts), C.POINTER(TmcOut)]
     L.tmc_hip_run.restype = C.c_int
+    L.tmc_hip_run_static.argtypes = [C.POINTER(TmcSpec), C.POINTER(TmcOpts), C.POINTER(TmcOut)]
+    L.tmc_hip_run_static.restype = C.c_int
+    L.tmc_static_scratch_bytes.restype = C.c_uint64
     L.tmc_last_error.restype = C.c_char_p
     L.tmc_fixed_point_unit.restype = C.c_double
     L.tmc_default_opts.argtypes = [C.POINTER(TmcOpts)]
@@ -299,8 +303,11 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
         arz: bool = False, profiles: bool = True, stream: int = 0, waves_per_group: int = 0,
         groups: int = 0, chunk: int = 0, arz_tile: tuple[int, int] | None = None,
         step_limit: int = 0, angles: bool = False, aggregate: bool = False,
-        time: tuple[int, float] | None = None) -> dict:
+        time: tuple[int, float] | None = None, no_alloc: bool = False) -> dict:
     """Simulate one run on the GPU; returns the reference's result schema plus extras.
+
+    With ``no_alloc`` the run goes through tmc_hip_run_static: no device allocation, every
+    device buffer from the library's static scratch array (same results).
 
     Extras: ``t_r``, ``a_rz`` (if ``arz``), the exit-angle profiles ``rd_a``/``t_a`` [na] and
     ``rd_ra``/``t_ra`` [nr, na] (if ``angles``), the time-of-flight profiles ``rd_t``/``t_t``
@@ -334,7 +341,8 @@ def run(spec: dict, *, photons: int = 0, batches: int = 10, seed: int = 1, devic
     tof = [np.zeros(max(int(time[0]), 1), np.uint64) for _ in range(2)] if time is not None else None
     if time is not None:
         out.rt, out.tt_t = _p(tof[0]), _p(tof[1])
-    code = L.tmc_hip_run(C.byref(cs), C.byref(o), C.byref(out))
+    entry = L.tmc_hip_run_static if no_alloc else L.tmc_hip_run
+    code = entry(C.byref(cs), C.byref(o), C.byref(out))
     if code != TMC_OK:
         _raise(code)
 
