@@ -182,34 +182,41 @@ def bench_batched(engine, steps: int, warmup: int, tune: bool) -> dict:
 
 
 def bench_surrogate(mc_photons_per_s: float | None = None) -> dict:
-    """The surrogate's kernels at full size: ensemble inference on 1.05e6 queries and a burst
-    of training steps (4 heads, 30 -> 4 x 512 GELU -> 202, batch 4096), device time by HIP
-    events; errors and the matched-error comparison come from the full pipeline run recorded in
-    profiles/surrogate_pipeline_full.json."""
+    """The surrogate at full size (4 heads, 30 -> 4 x 512 GELU -> 202): live on this GPU, the
+    inference kernels on 1.05e6 queries and a burst of training steps (batch 4096; HIP events),
+    and the end-to-end answer rate (stacks in host memory -> gated R_d / T_t in host memory:
+    packing, transfers, encoding, network and gate) on the inference workload's queries.
+    Accuracy comes from the full pipeline's recorded run (profiles/surrogate_pipeline_full.json:
+    training on 2^20 Monte Carlo stacks, judged by Monte Carlo at 1e7 photons), labelled as
+    recorded; batched Monte Carlo's rate at the matched error is this run's batched photons/s
+    divided by the photons per stack that the recorded error needs."""
     from tissuemc_amd.mlp import bench_kernels
+    from tissuemc_amd.surrogate import bench_answers
 
     out = bench_kernels(1_050_000, 200)
+    out["end_to_end"] = bench_answers()
     rep = ROOT / "profiles" / "surrogate_pipeline_full.json"
     if rep.exists():
         r = json.loads(rep.read_text())
         m = r["matched"]
+        j = r["judge"]
         out["recorded_pipeline"] = {
-            "source": "profiles/surrogate_pipeline_full.json",
-            "accepted_rd_median_rel_err": r["judge"]["rd_surrogate_accepted"]["median"],
-            "accepted_tt_median_rel_err": r["judge"]["tt_surrogate_accepted"]["median"],
+            "source": "profiles/surrogate_pipeline_full.json (recorded, not re-measured here)",
+            "accepted_rd_median_rel_err": j["rd_surrogate_accepted"]["median"],
+            "accepted_tt_median_rel_err": j["tt_surrogate_accepted"]["median"],
+            "accepted_rd_p90_rel_err": j["rd_surrogate_accepted"]["p90"],
+            "accepted_tt_p90_rel_err": j["tt_surrogate_accepted"]["p90"],
+            "accepted_rd_within_1pct": j["rd_surrogate_accepted"].get("within_1pct"),
+            "accepted_tt_within_1pct": j["tt_surrogate_accepted"].get("within_1pct"),
             "routed_frac": r["inference"]["routed_frac"],
-            "mc_stacks_per_s_at_matched_err": m["mc_stacks_per_s_matched"],
-            "speedup_in_domain_recorded": m["speedup_in_domain"],
-            "speedup_with_fallback_recorded": m["speedup_with_fallback"],
             "mc_photons_for_matched_err": m["mc_photons_for_matched_err"]}
-        out["speedup_in_domain_live"] = out["inference_stacks_per_s"] / m["mc_stacks_per_s_matched"]
         if mc_photons_per_s:
-            # batched MC at this run's measured rate, with the photons per stack it needs to match
-            # the surrogate's recorded median R_d error
-            mc_live = mc_photons_per_s / m["mc_photons_for_matched_err"]
-            out["mc_stacks_per_s_matched_live"] = mc_live
-            out["speedup_in_domain_live"] = out["inference_stacks_per_s"] / mc_live
-    log(f"bench: surrogate inference {out['inference_stacks_per_s']:.3g} stacks/s")
+            mc = mc_photons_per_s / m["mc_photons_for_matched_err"]
+            out["mc_stacks_per_s_matched_live"] = mc
+            out["speedup_vs_mc_device_only"] = out["inference_stacks_per_s"] / mc
+            out["speedup_vs_mc_end_to_end"] = out["end_to_end"]["answers_per_s_end_to_end"] / mc
+    log(f"bench: surrogate inference {out['inference_stacks_per_s']:.3g} stacks/s on the device, "
+        f"{out['end_to_end']['answers_per_s_end_to_end']:.3g} answers/s end to end")
     return out
 
 

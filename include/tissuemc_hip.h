@@ -281,6 +281,38 @@ int tmc_mlp_set_lr(tmc_mlp *model, float lr);
 /* Algorithmic matrix-product FLOPs of one training step and of one query (all heads). */
 int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query);
 
+/* Surrogate answers for layer stacks, end to end on the device (no reference counterpart).
+ * The network must have 30 inputs (5 layer slots x (log10 mua, log10 mus, g, n, log10 d), each
+ * mapped to [-1, 1] over gate->box, 0 in empty slots, then 5 slot masks) and >= 2 outputs, the
+ * first two being R_d and T_t; every output is log10(value + floor), standardized by y_mean and
+ * y_std. For each stack the call encodes the inputs on the device (in double precision, the
+ * host's arithmetic), checks the domain (layer count within the box's range, every present
+ * layer's five encoded parameters within the box, to 1e-9), runs the ensemble, and routes the
+ * stack to Monte Carlo (route = 1) if it is outside the domain or if the ensemble's spread of R_d
+ * relative to max(R_d, 1e-3), or of T_t relative to max(T_t, 1e-2), exceeds gate->tau. Stacks go
+ * through in chunks, the host packing the next chunk while the device works on the current one;
+ * only the answers come back. */
+typedef struct tmc_surrogate_gate {
+    tmc_stack_dist box;  /* the training distribution: encoding box and domain */
+    const float *y_mean; /* [outputs] */
+    const float *y_std;  /* [outputs] */
+    double floor;        /* outputs are log10(value + floor): 1e-7 in the pipeline */
+    double tau;          /* spread gate */
+} tmc_surrogate_gate;
+
+typedef struct tmc_surrogate_answers {
+    double *rd, *tt;               /* [n] ensemble R_d, T_t (>= 0) */
+    double *rd_spread, *tt_spread; /* [n] or NULL: ensemble standard deviation of R_d, T_t */
+    uint8_t *route;                /* [n] 1: answer with Monte Carlo */
+    uint8_t *in_domain;            /* [n] or NULL */
+    float *profiles;               /* [n x (outputs - 2)] or NULL: the other outputs' values */
+    double device_seconds;         /* the kernels (encode, network, gate), HIP events */
+    double wall_seconds;           /* the whole call: packing, transfers and kernels */
+} tmc_surrogate_answers;
+
+int tmc_mlp_answer(tmc_mlp *model, const tmc_surrogate_gate *gate, const tmc_stack *stacks,
+                   int64_t n, tmc_surrogate_answers *out);
+
 /* 1 if a HIP device is present and usable, else 0. */
 int tmc_hip_available(void);
 

@@ -31,6 +31,8 @@ MIRRORS = {
     "tmc_tune_entry": engine.TmcTuneEntry,
     "tmc_tune_report": engine.TmcTuneReport,
     "tmc_mlp_config": mlp.MlpConfig,
+    "tmc_surrogate_gate": mlp.SurrogateGate,
+    "tmc_surrogate_answers": mlp.SurrogateAnswers,
 }
 
 

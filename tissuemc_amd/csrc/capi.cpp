@@ -455,6 +455,19 @@ int tmc_mlp_predict(tmc_mlp *model, const float *x, int64_t n, float *mean, floa
     });
 }
 
+int tmc_mlp_answer(tmc_mlp *model, const tmc_surrogate_gate *gate, const tmc_stack *stacks,
+                   int64_t n, tmc_surrogate_answers *out) {
+    return guarded([&] {
+        TMC_NEED_MODEL();
+        if (!gate || !out) config_fail("gate and out must not be NULL");
+        if (n < 0) config_fail("n must be >= 0");
+        if (n > 0 && (!stacks || !out->rd || !out->tt || !out->route))
+            config_fail("stacks and out->rd, tt, route must not be NULL");
+        if (!gate->y_mean || !gate->y_std) config_fail("gate->y_mean and y_std must not be NULL");
+        tmc::mlp_answer(*model->impl, *gate, stacks, n, out);
+    });
+}
+
 int tmc_mlp_get_params(tmc_mlp *model, float *params) {
     return guarded([&] {
         TMC_NEED_MODEL();

@@ -60,6 +60,8 @@ void mlp_set_data(MlpModel &m, const float *x, const float *y, int64_t n, const 
 double mlp_train(MlpModel &m, int64_t steps, float *losses);
 void mlp_gradients(MlpModel &m, const int64_t *rows, float *grads, float *loss);
 double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *spread, float *per_head);
+void mlp_answer(MlpModel &m, const tmc_surrogate_gate &gate, const tmc_stack *stacks, int64_t n,
+                tmc_surrogate_answers *out);
 void mlp_get_params(MlpModel &m, float *out);
 void mlp_set_params(MlpModel &m, const float *in);
 void mlp_set_lr(MlpModel &m, float lr);

@@ -998,6 +998,111 @@ __global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int head
     }
 }
 
+// ---- surrogate answers: stacks in, gated R_d / T_t out
+
+constexpr int kSlots = 5;                         // layer slots of the encoding
+constexpr int kRawPerStack = 1 + kSlots * 5;      // layer count, then (mua, mus, g, n, d) per slot
+constexpr int kAnswerInputs = kSlots * 5 + kSlots;  // 25 encoded parameters + 5 slot masks
+
+struct EncodeArgs {
+    const double *raw;  // [Bq x kRawPerStack]
+    int rows, Bq, in_pad;
+    int min_layers, max_layers;
+    double center[5], half[5];  // encoding: (v - center) / half
+    double lo[5], hi[5];        // the domain box (log10 for mua, mus, d)
+    bf16 *X;                    // [Bq x in_pad]: the first 32 columns are written
+    uint8_t *dom;               // [Bq]
+};
+
+// One thread per query row: the 30 network inputs (in double, then float, then bf16, as the
+// host encoder and load_inputs do) and the domain flag. Rows past `rows` get zero inputs.
+__global__ __launch_bounds__(256) void encode_stacks(const EncodeArgs a) {
+    const int m = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (m >= a.Bq) return;
+    float f[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) f[j] = 0.0f;
+    bool ok = false;
+    if (m < a.rows) {
+        const double *r = a.raw + static_cast<long long>(m) * kRawPerStack;
+        const int nl = static_cast<int>(r[0]);
+        ok = nl >= a.min_layers && nl <= a.max_layers;
+        for (int i = 0; i < kSlots; ++i) {
+            if (i >= nl) continue;
+            const double *q = r + 1 + 5 * i;
+            const double v[5] = {q[0] > 0.0 ? log10(q[0]) : -99.0, q[1] > 0.0 ? log10(q[1]) : -99.0,
+                                 q[2], q[3], log10(q[4])};
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                f[5 * i + j] = static_cast<float>((v[j] - a.center[j]) / a.half[j]);
+                ok = ok && v[j] >= a.lo[j] - 1e-9 && v[j] <= a.hi[j] + 1e-9;
+            }
+            f[25 + i] = 1.0f;
+        }
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(a.X + static_cast<long long>(m) * a.in_pad);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = 8 * c + 2 * k;
+            w[k] = static_cast<uint32_t>(to_bf16(f[j])) | (static_cast<uint32_t>(to_bf16(f[j + 1])) << 16);
+        }
+        dst[c] = uint4{w[0], w[1], w[2], w[3]};
+    }
+    a.dom[m] = ok ? 1 : 0;
+}
+
+struct FinishArgs {
+    const float *mean;    // [rows x outputs] ensemble means (standardized)
+    const float *spread;  // [rows x 2] ensemble standard deviations of outputs 0, 1
+    const uint8_t *dom;
+    const float *y_mean, *y_std;  // [outputs]
+    int rows, outputs;
+    double floor, tau;
+    double *res;     // [rows x 4]: rd, tt, rd_spread, tt_spread
+    uint8_t *route;  // [rows]
+};
+
+// One thread per row: de-standardize R_d and T_t (float multiply, then add: no contraction to an
+// FMA, whose single rounding would differ from the host's), undo the
+// log10, and route the row to Monte Carlo if it is out of domain or its relative spread exceeds
+// tau (R_d relative to max(R_d, 1e-3), T_t to max(T_t, 1e-2)).
+__global__ __launch_bounds__(256) void finish_answers(const FinishArgs a) {
+#pragma clang fp contract(off)
+    const int m = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (m >= a.rows) return;
+    double v[2], sp[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const float y = a.mean[static_cast<long long>(m) * a.outputs + q] * a.y_std[q] + a.y_mean[q];
+        v[q] = fmax(exp10(static_cast<double>(y)) - a.floor, 0.0);
+        sp[q] = v[q] * 2.302585092994046 * static_cast<double>(a.spread[2 * m + q]) * static_cast<double>(a.y_std[q]);
+    }
+    const double rel_rd = sp[0] / fmax(fabs(v[0]), 1e-3);
+    const double rel_tt = sp[1] / fmax(fabs(v[1]), 1e-2);
+    double *o = a.res + 4ll * m;
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = sp[0];
+    o[3] = sp[1];
+    a.route[m] = (!a.dom[m] || rel_rd > a.tau || rel_tt > a.tau) ? 1 : 0;
+}
+
+// The other outputs' values, max(10^y - floor, 0), as float: one thread per (row, output).
+__global__ __launch_bounds__(256) void finish_profiles(const float *mean, const float *y_mean, const float *y_std,
+                                                       int rows, int outputs, double floor, float *prof) {
+#pragma clang fp contract(off)
+    const int width = outputs - 2;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(rows) * width) return;
+    const long long m = t / width;
+    const int c = static_cast<int>(t % width) + 2;
+    const float y = mean[m * outputs + c] * y_std[c] + y_mean[c];
+    prof[t] = static_cast<float>(fmax(exp10(static_cast<double>(y)) - floor, 0.0));
+}
+
 }  // namespace mlp
 
 #define MLP_TRY(call)                                                                          \
@@ -1121,8 +1226,14 @@ class MlpModel {
         if (graph_) (void)hipGraphDestroy(graph_);
         if (e0_) (void)hipEventDestroy(e0_);
         if (e1_) (void)hipEventDestroy(e1_);
+        for (int i = 0; i < 2; ++i) {
+            if (ans_k0_[i]) (void)hipEventDestroy(ans_k0_[i]);
+            if (ans_k1_[i]) (void)hipEventDestroy(ans_k1_[i]);
+            if (ans_done_[i]) (void)hipEventDestroy(ans_done_[i]);
+        }
         if (stream_) (void)hipStreamDestroy(stream_);
         for (void *q : allocs_) (void)hipFree(q);
+        for (void *q : pinned_) (void)hipHostFree(q);
     }
 
     long long logical_params() const {
@@ -1242,6 +1353,117 @@ class MlpModel {
             total += ms / 1000.0;
         }
         return total;
+    }
+
+    // Gated surrogate answers for stacks, end to end (tmc_mlp_answer): chunks of Bq_ stacks,
+    // the host packing chunk k + 1 into pinned memory while the device encodes, runs and gates
+    // chunk k; two pinned slots alternate.
+    void answer(const tmc_surrogate_gate &gate, const tmc_stack *stacks, int64_t n, tmc_surrogate_answers *out) {
+        const auto t_start = std::chrono::steady_clock::now();
+        if (cfg_.inputs != mlp::kAnswerInputs)
+            throw EngineError(TMC_ERR_CONFIG, "surrogate answers need a network with 30 inputs");
+        if (cfg_.outputs < 2) throw EngineError(TMC_ERR_CONFIG, "surrogate answers need >= 2 outputs");
+        const tmc_stack_dist &d = gate.box;
+        const double lo_lin[5] = {d.mua_lo, d.mus_lo, d.g_lo, d.n_lo, d.d_lo};
+        const double hi_lin[5] = {d.mua_hi, d.mus_hi, d.g_hi, d.n_hi, d.d_hi};
+        mlp::EncodeArgs ea{};
+        for (int j = 0; j < 5; ++j) {
+            const bool logp = j == 0 || j == 1 || j == 4;
+            if (!(hi_lin[j] > lo_lin[j]) || (logp && !(lo_lin[j] > 0.0)))
+                throw EngineError(TMC_ERR_CONFIG, "gate box: every range needs lo < hi (and lo > 0 for mua, mus, d)");
+            ea.lo[j] = logp ? std::log10(lo_lin[j]) : lo_lin[j];
+            ea.hi[j] = logp ? std::log10(hi_lin[j]) : hi_lin[j];
+            ea.center[j] = (ea.lo[j] + ea.hi[j]) / 2.0;
+            ea.half[j] = (ea.hi[j] - ea.lo[j]) / 2.0;
+        }
+        ea.min_layers = d.min_layers;
+        ea.max_layers = d.max_layers;
+        const bool want_prof = out->profiles != nullptr && cfg_.outputs > 2;
+        answer_buffers(want_prof);
+        MLP_TRY(hipMemcpyAsync(ans_ymean_, gate.y_mean, static_cast<size_t>(cfg_.outputs) * 4, hipMemcpyHostToDevice, stream_));
+        MLP_TRY(hipMemcpyAsync(ans_ystd_, gate.y_std, static_cast<size_t>(cfg_.outputs) * 4, hipMemcpyHostToDevice, stream_));
+        MLP_TRY(hipStreamSynchronize(stream_));
+
+        const int width = cfg_.outputs - 2;
+        const int64_t chunks = (n + Bq_ - 1) / Bq_;
+        double device = 0.0;
+        auto rows_of = [&](int64_t k) { return static_cast<int>(std::min<int64_t>(Bq_, n - k * Bq_)); };
+        // results of chunk k (its slot's events have completed) to the caller's arrays
+        auto drain = [&](int64_t k) {
+            const int slot = static_cast<int>(k & 1), rows = rows_of(k);
+            MLP_TRY(hipEventSynchronize(ans_done_[slot]));
+            float ms = 0.0f;
+            MLP_TRY(hipEventElapsedTime(&ms, ans_k0_[slot], ans_k1_[slot]));
+            device += ms / 1000.0;
+            const double *res = ans_pin_res_[slot];
+            const uint8_t *flags = reinterpret_cast<const uint8_t *>(res + 4ll * Bq_);
+            const int64_t r0 = k * Bq_;
+            for (int m = 0; m < rows; ++m) {
+                out->rd[r0 + m] = res[4 * m];
+                out->tt[r0 + m] = res[4 * m + 1];
+                if (out->rd_spread) out->rd_spread[r0 + m] = res[4 * m + 2];
+                if (out->tt_spread) out->tt_spread[r0 + m] = res[4 * m + 3];
+                out->route[r0 + m] = flags[m];
+                if (out->in_domain) out->in_domain[r0 + m] = flags[Bq_ + m];
+            }
+            if (want_prof)
+                std::memcpy(out->profiles + r0 * width, ans_pin_prof_[slot], static_cast<size_t>(rows) * width * 4);
+        };
+        for (int64_t k = 0; k < chunks; ++k) {
+            const int slot = static_cast<int>(k & 1), rows = rows_of(k);
+            if (k >= 2) drain(k - 2);
+            // pack: layer count, then the five parameters of the first kSlots layers (0 if absent)
+            double *raw = ans_pin_raw_[slot];
+            for (int m = 0; m < rows; ++m) {
+                const tmc_stack &st = stacks[k * Bq_ + m];
+                double *r = raw + static_cast<long long>(m) * mlp::kRawPerStack;
+                r[0] = static_cast<double>(st.num_layers);
+                for (int i = 0; i < mlp::kSlots; ++i) {
+                    double *q = r + 1 + 5 * i;
+                    if (i < st.num_layers && i < TMC_MAX_LAYERS) {
+                        const tmc_layer &L = st.layers[i];
+                        q[0] = L.mua; q[1] = L.mus; q[2] = L.g; q[3] = L.n; q[4] = L.d;
+                    } else {
+                        q[0] = q[1] = q[2] = q[3] = q[4] = 0.0;
+                    }
+                }
+            }
+            MLP_TRY(hipMemcpyAsync(ans_raw_, raw, static_cast<size_t>(rows) * mlp::kRawPerStack * 8,
+                                   hipMemcpyHostToDevice, stream_));
+            MLP_TRY(hipEventRecord(ans_k0_[slot], stream_));
+            ea.raw = ans_raw_;
+            ea.rows = rows;
+            ea.Bq = Bq_;
+            ea.in_pad = in_pad_;
+            ea.X = Xq_;
+            ea.dom = ans_dom_;
+            hipLaunchKernelGGL(mlp::encode_stacks, dim3((Bq_ + 255) / 256), dim3(256), 0, stream_, ea);
+            forward(Xq_, Bq_, Hq_, nullptr, true);
+            hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
+                               out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
+            mlp::FinishArgs fa{mean_, spread_, ans_dom_, ans_ymean_, ans_ystd_, rows, cfg_.outputs,
+                               gate.floor, gate.tau, ans_res_, ans_route_};
+            hipLaunchKernelGGL(mlp::finish_answers, dim3((rows + 255) / 256), dim3(256), 0, stream_, fa);
+            if (want_prof) {
+                const long long cells = static_cast<long long>(rows) * width;
+                hipLaunchKernelGGL(mlp::finish_profiles, dim3(static_cast<unsigned>((cells + 255) / 256)), dim3(256), 0,
+                                   stream_, mean_, ans_ymean_, ans_ystd_, rows, cfg_.outputs, gate.floor, ans_prof_);
+            }
+            MLP_TRY(hipGetLastError());
+            MLP_TRY(hipEventRecord(ans_k1_[slot], stream_));
+            double *res = ans_pin_res_[slot];
+            uint8_t *flags = reinterpret_cast<uint8_t *>(res + 4ll * Bq_);
+            MLP_TRY(hipMemcpyAsync(res, ans_res_, static_cast<size_t>(rows) * 32, hipMemcpyDeviceToHost, stream_));
+            MLP_TRY(hipMemcpyAsync(flags, ans_route_, static_cast<size_t>(rows), hipMemcpyDeviceToHost, stream_));
+            MLP_TRY(hipMemcpyAsync(flags + Bq_, ans_dom_, static_cast<size_t>(rows), hipMemcpyDeviceToHost, stream_));
+            if (want_prof)
+                MLP_TRY(hipMemcpyAsync(ans_pin_prof_[slot], ans_prof_, static_cast<size_t>(rows) * width * 4,
+                                       hipMemcpyDeviceToHost, stream_));
+            MLP_TRY(hipEventRecord(ans_done_[slot], stream_));
+        }
+        for (int64_t k = std::max<int64_t>(0, chunks - 2); k < chunks; ++k) drain(k);
+        out->device_seconds = device;
+        out->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     }
 
     void get_params(float *out) {
@@ -1570,6 +1792,43 @@ class MlpModel {
     std::vector<mlp::MatInfo> mats_;
     std::vector<int> real_in_, real_out_;
     std::vector<void *> allocs_;
+    // surrogate answers (allocated on first use): device buffers, pinned slots, events
+    double *ans_raw_ = nullptr, *ans_res_ = nullptr;
+    uint8_t *ans_dom_ = nullptr, *ans_route_ = nullptr;
+    float *ans_prof_ = nullptr, *ans_ymean_ = nullptr, *ans_ystd_ = nullptr;
+    double *ans_pin_raw_[2] = {nullptr, nullptr}, *ans_pin_res_[2] = {nullptr, nullptr};
+    float *ans_pin_prof_[2] = {nullptr, nullptr};
+    hipEvent_t ans_k0_[2] = {nullptr, nullptr}, ans_k1_[2] = {nullptr, nullptr}, ans_done_[2] = {nullptr, nullptr};
+    std::vector<void *> pinned_;
+
+    void answer_buffers(bool profiles) {
+        if (!ans_raw_) {
+            ans_raw_ = alloc<double>(static_cast<long long>(Bq_) * mlp::kRawPerStack);
+            ans_res_ = alloc<double>(4ll * Bq_);
+            ans_dom_ = alloc<uint8_t>(Bq_);
+            ans_route_ = alloc<uint8_t>(Bq_);
+            ans_ymean_ = alloc<float>(cfg_.outputs);
+            ans_ystd_ = alloc<float>(cfg_.outputs);
+            for (int i = 0; i < 2; ++i) {
+                ans_pin_raw_[i] = pinned<double>(static_cast<size_t>(Bq_) * mlp::kRawPerStack);
+                ans_pin_res_[i] = pinned<double>(4 * static_cast<size_t>(Bq_) + (2 * static_cast<size_t>(Bq_) + 7) / 8);
+                MLP_TRY(hipEventCreate(&ans_k0_[i]));
+                MLP_TRY(hipEventCreate(&ans_k1_[i]));
+                MLP_TRY(hipEventCreateWithFlags(&ans_done_[i], hipEventDisableTiming));
+            }
+        }
+        if (profiles && !ans_prof_) {
+            const size_t cells = static_cast<size_t>(Bq_) * (cfg_.outputs - 2);
+            ans_prof_ = alloc<float>(static_cast<long long>(cells));
+            for (int i = 0; i < 2; ++i) ans_pin_prof_[i] = pinned<float>(cells);
+        }
+    }
+    template <class T> T *pinned(size_t n) {
+        void *q = nullptr;
+        MLP_TRY(hipHostMalloc(&q, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+        pinned_.push_back(q);
+        return static_cast<T *>(q);
+    }
     int splits_ = 1;
     float *bpart_ = nullptr;  // bias-gradient partials [H x B / 64 x nb_]
     int nb_ = 0;              // biases per head
@@ -1611,6 +1870,10 @@ double mlp_train(MlpModel &m, int64_t steps, float *losses) { return m.train(ste
 void mlp_gradients(MlpModel &m, const int64_t *rows, float *grads, float *loss) { m.gradients(rows, grads, loss); }
 double mlp_predict(MlpModel &m, const float *x, int64_t n, float *mean, float *spread, float *per_head) {
     return m.predict(x, n, mean, spread, per_head);
+}
+void mlp_answer(MlpModel &m, const tmc_surrogate_gate &gate, const tmc_stack *stacks, int64_t n,
+                tmc_surrogate_answers *out) {
+    m.answer(gate, stacks, n, out);
 }
 void mlp_get_params(MlpModel &m, float *out) { m.get_params(out); }
 void mlp_set_params(MlpModel &m, const float *in) { m.set_params(in); }

@@ -115,7 +115,7 @@ EXPORTED = ("tmc_hip_available", "tmc_hip_run", "tmc_last_error", "tmc_fixed_poi
             "tmc_mlp_param_count", "tmc_mlp_set_data", "tmc_mlp_train", "tmc_mlp_gradients",
             "tmc_mlp_predict", "tmc_mlp_get_params", "tmc_mlp_set_params", "tmc_mlp_set_lr",
             "tmc_mlp_flops", "tmc_autotune", "tmc_batched_autotune", "tmc_hip_run_static",
-            "tmc_static_scratch_bytes")
+            "tmc_static_scratch_bytes", "tmc_mlp_answer")
 
 _lib: C.CDLL | None = None
 

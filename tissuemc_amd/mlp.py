@@ -20,6 +20,21 @@ class MlpConfig(C.Structure):
                 ("weight_decay", C.c_float), ("seed", C.c_uint64), ("device", C.c_int32)]
 
 
+class SurrogateGate(C.Structure):
+    _fields_ = [("box", engine.TmcStackDist), ("y_mean", C.POINTER(C.c_float)),
+                ("y_std", C.POINTER(C.c_float)), ("floor", C.c_double), ("tau", C.c_double)]
+
+
+_PD = C.POINTER(C.c_double)
+_PU8 = C.POINTER(C.c_uint8)
+
+
+class SurrogateAnswers(C.Structure):
+    _fields_ = [("rd", _PD), ("tt", _PD), ("rd_spread", _PD), ("tt_spread", _PD),
+                ("route", _PU8), ("in_domain", _PU8), ("profiles", C.POINTER(C.c_float)),
+                ("device_seconds", C.c_double), ("wall_seconds", C.c_double)]
+
+
 _PF = C.POINTER(C.c_float)
 _bound = False
 
@@ -52,6 +67,9 @@ def _lib():
         L.tmc_mlp_set_lr.restype = C.c_int
         L.tmc_mlp_flops.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.tmc_mlp_flops.restype = C.c_int
+        L.tmc_mlp_answer.argtypes = [C.c_void_p, C.POINTER(SurrogateGate), C.POINTER(engine.TmcStack),
+                                     C.c_int64, C.POINTER(SurrogateAnswers)]
+        L.tmc_mlp_answer.restype = C.c_int
         _bound = True
     return L
 
@@ -117,6 +135,35 @@ class Mlp:
         self._check(_lib().tmc_mlp_predict(self._h, _f(x), n, _f(mean), _f(spread), _f(ph),
                                            C.byref(sec)))
         return mean, spread, ph, sec.value
+
+    def answer(self, stacks, *, y_mean: np.ndarray, y_std: np.ndarray, tau: float,
+               floor: float = 1e-7, box: engine.TmcStackDist | None = None,
+               profiles: bool = False) -> dict:
+        """Gated surrogate answers for stacks, end to end on the device (tmc_mlp_answer): R_d,
+        T_t, their ensemble spreads, the route-to-Monte-Carlo and in-domain flags, optionally the
+        other outputs' values, and the device and wall-clock seconds of the call."""
+        arr = engine.stacks_array(stacks)
+        n = len(arr)
+        ym = np.ascontiguousarray(y_mean, np.float32)
+        ys = np.ascontiguousarray(y_std, np.float32)
+        assert ym.shape == ys.shape == (self.outputs,)
+        gate = SurrogateGate(box=box if box is not None else engine.default_stack_dist(),
+                             y_mean=_f(ym), y_std=_f(ys), floor=floor, tau=tau)
+        res = {k: np.zeros(n) for k in ("rd", "tt", "rd_spread", "tt_spread")}
+        route = np.zeros(n, np.uint8)
+        dom = np.zeros(n, np.uint8)
+        prof = np.zeros((n, self.outputs - 2), np.float32) if profiles else None
+        out = SurrogateAnswers(rd=res["rd"].ctypes.data_as(_PD), tt=res["tt"].ctypes.data_as(_PD),
+                               rd_spread=res["rd_spread"].ctypes.data_as(_PD),
+                               tt_spread=res["tt_spread"].ctypes.data_as(_PD),
+                               route=route.ctypes.data_as(_PU8), in_domain=dom.ctypes.data_as(_PU8),
+                               profiles=_f(prof))
+        self._check(_lib().tmc_mlp_answer(self._h, C.byref(gate), arr, n, C.byref(out)))
+        res.update(route=route.astype(bool), in_domain=dom.astype(bool),
+                   device_seconds=out.device_seconds, wall_seconds=out.wall_seconds)
+        if profiles:
+            res["profiles"] = prof
+        return res
 
     def get_params(self) -> np.ndarray:
         p = np.zeros((self.heads, self.param_count), np.float32)

@@ -19,6 +19,7 @@ their errors are learned (and gated) in relative terms.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import math
 import sys
@@ -54,18 +55,39 @@ def box(dist) -> np.ndarray:
                      [math.log10(dist.d_lo), math.log10(dist.d_hi)]])
 
 
+_LAYER_DT = np.dtype([(k, "<f8") for k in ("n", "mua", "mus", "g", "d")])
+
+
+def stack_view(stacks) -> np.ndarray:
+    """A numpy structured view (no copy) of a ctypes array of engine.TmcStack."""
+    arr = engine.stacks_array(stacks)
+    T = engine.TmcStack
+    dt = np.dtype({"names": ["num_layers", "n_above", "n_below", "layers", "photons"],
+                   "formats": ["<i4", "<f8", "<f8", (_LAYER_DT, (engine.MAX_LAYERS,)), "<u8"],
+                   "offsets": [T.num_layers.offset, T.n_above.offset, T.n_below.offset,
+                               T.layers.offset, T.photons.offset],
+                   "itemsize": C.sizeof(T)})
+    return np.frombuffer(arr, dtype=dt, count=len(arr))
+
+
 def stack_params(stacks) -> tuple[np.ndarray, np.ndarray]:
     """Raw parameters [n x 5 slots x 5] (log10 for mua, mus, d; 0 in empty slots) and layer counts."""
-    n = len(stacks)
-    raw = np.zeros((n, SLOTS, 5))
-    nl = np.zeros(n, np.int64)
-    for k in range(n):
-        s = stacks[k]
-        nl[k] = s.num_layers
-        for i in range(min(s.num_layers, SLOTS)):
-            L = s.layers[i]
-            raw[k, i] = (math.log10(L.mua) if L.mua > 0 else -99.0, math.log10(L.mus) if L.mus > 0 else -99.0,
-                         L.g, L.n, math.log10(L.d))
+    from numpy.lib import recfunctions
+
+    v = stack_view(stacks)
+    nl = v["num_layers"].astype(np.int64)
+    # (n, mua, mus, g, d) of the first SLOTS layers as one dense [n x SLOTS x 5] array
+    L = recfunctions.structured_to_unstructured(v["layers"][:, :SLOTS])
+    present = np.arange(SLOTS)[None, :] < nl[:, None]
+    raw = np.zeros((len(nl), SLOTS, 5))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for j, (col, log) in enumerate(((1, True), (2, True), (3, False), (0, False), (4, True))):
+            x = L[:, :, col]
+            if log:
+                absent = -99.0 if j < 2 else 0.0
+                x = np.where(x > 0, np.log10(np.where(x > 0, x, 1.0)), absent)
+            raw[:, :, j] = x
+    raw[~present] = 0.0
     return raw, nl
 
 
@@ -204,6 +226,12 @@ class Surrogate:
                 "tt_spread": tt * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1],
                 "seconds": sec}
 
+    def answer(self, stacks, tau: float, profiles: bool = False) -> dict:
+        """Gated answers end to end on the device (encoding, domain gate, ensemble, spread gate:
+        tmc_mlp_answer); the same decisions as predict() + gate() on features()/in_domain()."""
+        return self.net.answer(stacks, y_mean=self.y_mean, y_std=self.y_std, tau=tau,
+                               floor=FLOOR, box=self.dist, profiles=profiles)
+
     def save(self, path: Path) -> None:
         np.savez(path, params=self.net.get_params(), y_mean=self.y_mean, y_std=self.y_std)
 
@@ -233,6 +261,52 @@ def summarize(err: np.ndarray) -> dict:
         return {"n": 0}
     return {"n": int(len(err)), "median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
             "mean": float(err.mean()), "within_1pct": float(np.mean(err <= 0.01))}
+
+
+def query_stacks(n_in: int, ood_frac: float, seed: int):
+    """The inference workload's queries: n_in stacks of the training distribution, then
+    round(n_in * ood_frac) out-of-range stacks (mus up to 1000 /cm, thickness up to 1 cm)."""
+    n_ood = int(round(n_in * ood_frac))
+    q_in = engine.generate_stacks(seed + 1, 0, n_in)
+    q_ood = engine.generate_stacks(seed + 2, 0, n_ood, engine.default_stack_dist(mus_hi=1000.0, d_hi=1.0))
+    out = (engine.TmcStack * (n_in + n_ood))()
+    C.memmove(out, q_in, C.sizeof(q_in))
+    C.memmove(C.byref(out, C.sizeof(q_in)), q_ood, C.sizeof(q_ood))
+    return out
+
+
+def bench_answers(n_in: int = 1_000_000, ood_frac: float = 0.05, tau: float = 0.02,
+                  repeats: int = 3) -> dict:
+    """End-to-end answer rate of the full-size network (4 heads, 30 -> 4 x 512 -> 202; random
+    weights, which do not change the work) on the inference workload's queries: stacks in host
+    memory -> gated R_d / T_t in host memory. The device path (tmc_mlp_answer: packing, transfers,
+    encoding, network, gate) beside the host path it replaces (numpy encoding and domain gate,
+    tmc_mlp_predict with every output returned, numpy de-standardization and gate)."""
+    qs = query_stacks(n_in, ood_frac, seed=1)
+    n = len(qs)
+    m = Surrogate(seed=3)
+    rng = np.random.default_rng(3)
+    m.y_mean = rng.uniform(-2.0, -0.5, N_OUT).astype(np.float32)
+    m.y_std = rng.uniform(0.2, 0.6, N_OUT).astype(np.float32)
+    try:
+        m.answer(engine.stacks_array([qs[i] for i in range(2000)]), tau)  # warm-up
+        runs = [m.answer(qs, tau) for _ in range(repeats)]
+        best = min(runs, key=lambda r: r["wall_seconds"])
+        t0 = time.perf_counter()
+        X, dom = features(qs, m.dist), in_domain(qs, m.dist)
+        t_enc = time.perf_counter() - t0
+        p = m.predict(X)
+        route = gate(p, dom, tau)
+        t_host = time.perf_counter() - t0
+        agree = float(np.mean(route == best["route"]))
+    finally:
+        m.close()
+    return {"queries": n, "tau": tau,
+            "answers_per_s_end_to_end": n / best["wall_seconds"],
+            "answer_wall_seconds": best["wall_seconds"], "answer_device_seconds": best["device_seconds"],
+            "routed_frac": float(best["route"].mean()),
+            "host_path_seconds": t_host, "host_path_encoding_seconds": t_enc,
+            "host_path_answers_per_s": n / t_host, "route_agreement_with_host_path": agree}
 
 
 # ------------------------------------------------------------------ the pipeline
