@@ -288,8 +288,9 @@ int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query)
  * y_std. For each stack the call encodes the inputs on the device (in double precision, the
  * host's arithmetic), checks the domain (layer count within the box's range, every present
  * layer's five encoded parameters within the box, to 1e-9), runs the ensemble, and routes the
- * stack to Monte Carlo (route = 1) if it is outside the domain or if the ensemble's spread of R_d
- * relative to max(R_d, 1e-3), or of T_t relative to max(T_t, 1e-2), exceeds gate->tau. Stacks go
+ * stack to Monte Carlo (route = 1) if it is outside the domain, if the ensemble's spread of R_d
+ * relative to max(R_d, 1e-3) exceeds gate->tau_rd, or if that of T_t relative to max(T_t, 1e-2)
+ * exceeds gate->tau_tt. Stacks go
  * through in chunks, the host packing the next chunk while the device works on the current one;
  * only the answers come back. */
 typedef struct tmc_surrogate_gate {
@@ -297,7 +298,8 @@ typedef struct tmc_surrogate_gate {
     const float *y_mean; /* [outputs] */
     const float *y_std;  /* [outputs] */
     double floor;        /* outputs are log10(value + floor): 1e-7 in the pipeline */
-    double tau;          /* spread gate */
+    double tau_rd;       /* spread gate of R_d */
+    double tau_tt;       /* spread gate of T_t */
 } tmc_surrogate_gate;
 
 typedef struct tmc_surrogate_answers {

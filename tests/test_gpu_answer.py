@@ -44,9 +44,10 @@ def test_device_answers_match_the_host_path(model):
     qs = queries(36_000, 4_000)  # three chunks of 16384 rows, the last one partial
     dom = S.in_domain(qs)
     p = model.predict(S.features(qs))
-    rel = np.maximum(p["rd_spread"] / np.maximum(p["rd"], 1e-3), p["tt_spread"] / np.maximum(p["tt"], 1e-2))
-    tau = float(np.quantile(rel[dom], 0.7))
+    rel_rd, rel_tt = S.relative_spreads(p)
+    tau = (float(np.quantile(rel_rd[dom], 0.8)), float(np.quantile(rel_tt[dom], 0.7)))
     route = S.gate(p, dom, tau)
+    assert 0.2 < np.mean(route[dom]) < 0.6
     d = model.answer(qs, tau, profiles=True)
     assert np.array_equal(d["in_domain"], dom)
     assert 0 < (~dom).sum() < 4_000 + 100

@@ -52,3 +52,39 @@ def test_vectorised_encoding_matches_the_definition():
     assert np.allclose(X, Xr, rtol=0, atol=1e-7)
     assert np.array_equal(dom, dr)
     assert dom[:300].all() and 0 < (~dom[300:]).sum() < 300 and not dom[-1]
+
+
+def test_gate_and_threshold_choice():
+    """Per-output gate: a stack is routed when it is out of domain or either relative spread
+    exceeds its threshold; the chosen pair meets the target at the 90th percentile of both
+    outputs' held-out errors and accepts at least as many stacks as any single threshold that
+    does."""
+    rng = np.random.default_rng(1)
+    n = 4000
+    rd = rng.uniform(0.01, 0.5, n)
+    tt = rng.uniform(0.0, 0.8, n)
+    s_rd = rng.uniform(0.0, 0.02, n) * rd           # relative spreads up to 2% ...
+    s_tt = rng.uniform(0.0, 0.05, n) * np.maximum(tt, 1e-2)  # ... and 5%
+    pred = {"rd": rd, "tt": tt, "rd_spread": s_rd, "tt_spread": s_tt}
+    dom = rng.random(n) > 0.05
+    rel_rd, rel_tt = S.relative_spreads(pred)
+    # errors that grow with the spreads: the gate can reach the target
+    e_rd, e_tt = 0.2 * rel_rd, 0.1 * rel_tt
+    route = S.gate(pred, dom, (0.01, 0.02))
+    assert np.array_equal(route, ~dom | (rel_rd > 0.01) | (rel_tt > 0.02))
+    a, b = S.choose_gate(pred, dom, e_rd, e_tt, target=0.003, max_route=0.1, min_accept=0.3)
+    acc = ~S.gate(pred, dom, (a, b))
+    assert np.quantile(e_rd[acc], 0.9) <= 0.003 and np.quantile(e_tt[acc], 0.9) <= 0.003
+    assert acc.sum() >= 0.3 * n
+    assert a < b  # T_t's spreads are wider: it needs the looser threshold
+    # a single threshold for both outputs meeting the target accepts fewer stacks
+    grid = np.geomspace(1e-4, 1.0, 61)
+    single = []
+    for t in grid:
+        keep = ~S.gate(pred, dom, t)
+        if keep.any() and np.quantile(e_rd[keep], 0.9) <= 0.003 and np.quantile(e_tt[keep], 0.9) <= 0.003:
+            single.append(int(keep.sum()))
+    assert max(single) < acc.sum()
+    # unreachable target: one threshold routing at most max_route
+    a, b = S.choose_gate(pred, dom, e_rd + 1.0, e_tt, target=0.004, max_route=0.2)
+    assert a == b and np.mean(S.gate(pred, dom, a)) <= 0.2

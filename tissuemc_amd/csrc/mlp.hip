@@ -1060,15 +1060,16 @@ struct FinishArgs {
     const uint8_t *dom;
     const float *y_mean, *y_std;  // [outputs]
     int rows, outputs;
-    double floor, tau;
+    double floor, tau_rd, tau_tt;
     double *res;     // [rows x 4]: rd, tt, rd_spread, tt_spread
     uint8_t *route;  // [rows]
 };
 
 // One thread per row: de-standardize R_d and T_t (float multiply, then add: no contraction to an
 // FMA, whose single rounding would differ from the host's), undo the
-// log10, and route the row to Monte Carlo if it is out of domain or its relative spread exceeds
-// tau (R_d relative to max(R_d, 1e-3), T_t to max(T_t, 1e-2)).
+// log10, and route the row to Monte Carlo if it is out of domain or a relative spread exceeds
+// its threshold (R_d's relative to max(R_d, 1e-3) tau_rd, T_t's relative to max(T_t, 1e-2)
+// tau_tt).
 __global__ __launch_bounds__(256) void finish_answers(const FinishArgs a) {
 #pragma clang fp contract(off)
     const int m = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1087,7 +1088,7 @@ __global__ __launch_bounds__(256) void finish_answers(const FinishArgs a) {
     o[1] = v[1];
     o[2] = sp[0];
     o[3] = sp[1];
-    a.route[m] = (!a.dom[m] || rel_rd > a.tau || rel_tt > a.tau) ? 1 : 0;
+    a.route[m] = (!a.dom[m] || rel_rd > a.tau_rd || rel_tt > a.tau_tt) ? 1 : 0;
 }
 
 // The other outputs' values, max(10^y - floor, 0), as float: one thread per (row, output).
@@ -1442,7 +1443,7 @@ class MlpModel {
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             mlp::FinishArgs fa{mean_, spread_, ans_dom_, ans_ymean_, ans_ystd_, rows, cfg_.outputs,
-                               gate.floor, gate.tau, ans_res_, ans_route_};
+                               gate.floor, gate.tau_rd, gate.tau_tt, ans_res_, ans_route_};
             hipLaunchKernelGGL(mlp::finish_answers, dim3((rows + 255) / 256), dim3(256), 0, stream_, fa);
             if (want_prof) {
                 const long long cells = static_cast<long long>(rows) * width;

@@ -226,7 +226,7 @@ class Surrogate:
                 "tt_spread": tt * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1],
                 "seconds": sec}
 
-    def answer(self, stacks, tau: float, profiles: bool = False) -> dict:
+    def answer(self, stacks, tau, profiles: bool = False) -> dict:
         """Gated answers end to end on the device (encoding, domain gate, ensemble, spread gate:
         tmc_mlp_answer); the same decisions as predict() + gate() on features()/in_domain()."""
         return self.net.answer(stacks, y_mean=self.y_mean, y_std=self.y_std, tau=tau,
@@ -244,12 +244,46 @@ class Surrogate:
         self.net.close()
 
 
-def gate(pred: dict, domain: np.ndarray, tau: float) -> np.ndarray:
-    """True where the query goes back to Monte Carlo: out of the training box, or ensemble
-    spread above tau relative to the predicted R_d or T_t (T_t relative to max(T_t, 0.01))."""
-    rel_rd = pred["rd_spread"] / np.maximum(np.abs(pred["rd"]), 1e-3)
-    rel_tt = pred["tt_spread"] / np.maximum(np.abs(pred["tt"]), 1e-2)
-    return (~domain) | (rel_rd > tau) | (rel_tt > tau)
+def relative_spreads(pred: dict) -> tuple[np.ndarray, np.ndarray]:
+    """Ensemble spread of R_d relative to max(R_d, 1e-3) and of T_t relative to max(T_t, 1e-2)."""
+    return (pred["rd_spread"] / np.maximum(np.abs(pred["rd"]), 1e-3),
+            pred["tt_spread"] / np.maximum(np.abs(pred["tt"]), 1e-2))
+
+
+def gate(pred: dict, domain: np.ndarray, tau) -> np.ndarray:
+    """True where the query goes back to Monte Carlo: out of the training box, or a relative
+    ensemble spread above its threshold. `tau` is one threshold for both outputs or a pair
+    (R_d's, T_t's)."""
+    tau_rd, tau_tt = (tau, tau) if np.isscalar(tau) else tau
+    rel_rd, rel_tt = relative_spreads(pred)
+    return (~domain) | (rel_rd > tau_rd) | (rel_tt > tau_tt)
+
+
+def choose_gate(pred: dict, domain: np.ndarray, err_rd: np.ndarray, err_tt: np.ndarray,
+                target: float, max_route: float, min_accept: float = 0.5) -> tuple[float, float]:
+    """Per-output thresholds from held-out stacks: of the (tau_rd, tau_tt) pairs on a grid whose
+    accepted stacks have a 90th-percentile R_d error and T_t error both within `target` (and
+    accept at least `min_accept` of them), the pair accepting the most; if no pair meets the
+    target, the single threshold for both outputs that routes `max_route` of them."""
+    rel_rd, rel_tt = relative_spreads(pred)
+    grid = np.geomspace(1e-4, 1.0, 61)
+    n = len(domain)
+    best, best_acc = None, -1
+    for a in grid:
+        ok_rd = domain & (rel_rd <= a)
+        if ok_rd.sum() < min_accept * n:
+            continue
+        for b in grid:
+            acc = ok_rd & (rel_tt <= b)
+            k = int(acc.sum())
+            if k < min_accept * n or k <= best_acc:
+                continue
+            if np.quantile(err_rd[acc], 0.9) <= target and np.quantile(err_tt[acc], 0.9) <= target:
+                best, best_acc = (float(a), float(b)), k
+    if best is not None:
+        return best
+    t = float(next((t for t in grid if np.mean(gate(pred, domain, t)) <= max_route), 1.0))
+    return t, t
 
 
 def rel_err(pred: np.ndarray, ref: np.ndarray, floor: float) -> np.ndarray:
@@ -337,53 +371,50 @@ def pipeline(args) -> dict:
         report["train"] = fit
         model.save(out / "model.npz")
 
-        # choose the gate threshold on the held-out stacks: the largest tau (from a grid) whose
-        # accepted held-out stacks have a 90th-percentile R_d and T_t error within the target
+        # per-output gate thresholds chosen on the held-out stacks
         pv = model.predict(X[:n_val])
         dom = in_domain([stacks[i] for i in range(n_val)], dist)
-        err_v = np.maximum(rel_err(pv["rd"], mc.rd[:n_val], 1e-3), rel_err(pv["tt"], mc.tt[:n_val], 1e-2))
-        tau = args.tau
-        if tau <= 0:
-            # the largest threshold whose accepted held-out stacks meet the target at the 90th
-            # percentile; if none does, the threshold that routes args.max_route of them
-            grid = np.geomspace(1e-4, 1.0, 80)
-            ok = [t for t in grid if (~gate(pv, dom, t)).sum() >= 0.5 * n_val
-                  and np.quantile(err_v[~gate(pv, dom, t)], 0.9) <= args.target_err]
-            if ok:
-                tau = float(max(ok))
-            else:
-                tau = float(next((t for t in grid if np.mean(gate(pv, dom, t)) <= args.max_route), 1.0))
-        report["gate"] = {"tau": tau, "validation_stacks": n_val,
-                          "validation_accept": float(np.mean(~gate(pv, dom, tau))),
-                          "validation_err_all": summarize(err_v)}
-        log(f"gate: tau {tau:.4g}, validation accepts {report['gate']['validation_accept']:.3f}")
+        e_rd_v, e_tt_v = rel_err(pv["rd"], mc.rd[:n_val], 1e-3), rel_err(pv["tt"], mc.tt[:n_val], 1e-2)
+        if args.tau > 0:
+            tau = (args.tau, args.tau)
+        else:
+            tau = choose_gate(pv, dom, e_rd_v, e_tt_v, args.target_err, args.max_route)
+        acc_v = ~gate(pv, dom, tau)
+        report["gate"] = {"tau_rd": tau[0], "tau_tt": tau[1], "validation_stacks": n_val,
+                          "validation_accept": float(np.mean(acc_v)),
+                          "validation_rd_accepted": summarize(e_rd_v[acc_v]),
+                          "validation_tt_accepted": summarize(e_tt_v[acc_v]),
+                          "validation_err_all": summarize(np.maximum(e_rd_v, e_tt_v))}
+        log(f"gate: tau_rd {tau[0]:.4g}, tau_tt {tau[1]:.4g}, validation accepts "
+            f"{report['gate']['validation_accept']:.3f}")
 
-        # queries: the training distribution plus out-of-range stacks
-        n_ood = int(round(args.queries * args.ood_frac))
-        q_in = engine.generate_stacks(args.seed + 1, 0, args.queries)
-        q_ood = engine.generate_stacks(args.seed + 2, 0, n_ood,
-                                       engine.default_stack_dist(mus_hi=1000.0, d_hi=1.0))
-        queries = (engine.TmcStack * (args.queries + n_ood))()
-        for i in range(args.queries):
-            queries[i] = q_in[i]
-        for i in range(n_ood):
-            queries[args.queries + i] = q_ood[i]
+        # queries: the training distribution plus out-of-range stacks, answered end to end on
+        # the device (encoding, domain gate, ensemble, spread gate: tmc_mlp_answer)
+        queries = query_stacks(args.queries, args.ood_frac, args.seed)
         nq = len(queries)
+        model.answer(engine.stacks_array([queries[i] for i in range(2000)]), tau)  # warm-up
+        ans = model.answer(queries, tau)
+        route, dom_q = ans["route"], ans["in_domain"]
+        pq = {"rd": ans["rd"], "tt": ans["tt"]}
+        # the host path on the same queries (numpy encoding and gates around tmc_mlp_predict)
         t0 = time.perf_counter()
-        Xq = features(queries, dist)
-        dom_q = in_domain(queries, dist)
-        t_feat = time.perf_counter() - t0
-        pq = model.predict(Xq)
-        route = gate(pq, dom_q, tau)
-        report["inference"] = {"queries": nq, "device_seconds": pq["seconds"],
-                               "stacks_per_s_device": nq / pq["seconds"],
-                               "feature_seconds_host": t_feat,
-                               "tflops": per_query * nq / pq["seconds"] / 1e12,
-                               "mfma_frac_bf16": per_query * nq / pq["seconds"] / 2.5e15,
+        host = model.predict(features(queries, dist))
+        host_route = gate(host, in_domain(queries, dist), tau)
+        t_host = time.perf_counter() - t0
+        pq["az"] = host["az"]
+        report["inference"] = {"queries": nq, "device_seconds": ans["device_seconds"],
+                               "wall_seconds": ans["wall_seconds"],
+                               "stacks_per_s_device": nq / ans["device_seconds"],
+                               "stacks_per_s_end_to_end": nq / ans["wall_seconds"],
+                               "network_device_seconds": host["seconds"],
+                               "tflops": per_query * nq / host["seconds"] / 1e12,
+                               "mfma_frac_bf16": per_query * nq / host["seconds"] / 2.5e15,
+                               "host_path_seconds": t_host,
+                               "host_path_route_agreement": float(np.mean(host_route == route)),
                                "routed": int(route.sum()), "routed_frac": float(route.mean()),
                                "out_of_domain": int((~dom_q).sum())}
-        log(f"inference: {nq} queries, {pq['seconds'] * 1e3:.1f} ms on the device, "
-            f"{int(route.sum())} routed to MC")
+        log(f"inference: {nq} queries, {ans['wall_seconds'] * 1e3:.1f} ms end to end "
+            f"({ans['device_seconds'] * 1e3:.1f} ms on the device), {int(route.sum())} routed to MC")
 
         # fallback: Monte Carlo at args.fallback_photons for the routed queries
         routed_idx = np.nonzero(route)[0]
@@ -429,12 +460,14 @@ def pipeline(args) -> dict:
         mc_rate = report["data"]["mc_photons_per_s"]
         mc_stacks_per_s = mc_rate / n_match
         fb_cost = (report["fallback"]["kernel_seconds"])
-        total_s = pq["seconds"] + fb_cost
+        total_s = ans["wall_seconds"] + fb_cost
         report["matched"] = {
             "surrogate_median_rd_err": eps, "mc_photons_for_matched_err": n_match,
             "mc_stacks_per_s_matched": mc_stacks_per_s,
-            "surrogate_stacks_per_s_in_domain": nq / pq["seconds"],
-            "speedup_in_domain": (nq / pq["seconds"]) / mc_stacks_per_s,
+            "surrogate_stacks_per_s_device": nq / ans["device_seconds"],
+            "surrogate_stacks_per_s_end_to_end": nq / ans["wall_seconds"],
+            "speedup_device_only": (nq / ans["device_seconds"]) / mc_stacks_per_s,
+            "speedup_end_to_end": (nq / ans["wall_seconds"]) / mc_stacks_per_s,
             "answered_stacks_per_s_with_fallback": nq / total_s,
             "speedup_with_fallback": (nq / total_s) / mc_stacks_per_s,
         }
