@@ -127,10 +127,18 @@ struct UnitRec {
 };
 static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
 
+// The rows a wave's slot is flushed into (ColdRows order): read from device memory when a unit
+// ends, so that the step loop does not keep these pointers in scalar registers.
+enum ColdRow { kColdRd = 0, kColdTt, kColdEvents, kColdAnomalies, kColdBsteps, kColdAbsorbed, kColdAz, kColdRows };
+static_assert(kColdRd == kSlotRd && kColdTt == kSlotTt && kColdEvents == kSlotEvents &&
+                  kColdAnomalies == kSlotAnomalies && kColdBsteps == kSlotBoundary && kColdAbsorbed == kSlotAbs,
+              "the slot's scalars and the flush rows share their order");
+
 struct KernelArgs {
     const UnitRec *units;
     const float *stacks;
     const uint64_t *key;  // [1] seed, in device memory so that a captured graph can be replayed
+    uint64_t *const *cold;  // [kColdRows] the per-batch rows below, for flush_slot
     uint64_t *next;       // [U] ids claimed so far in each unit
     uint64_t *cursor;     // [1] units handed out so far
     uint64_t *rd, *tt, *absorbed, *events, *anomalies, *bsteps, *az, *rr, *tr, *arz;
@@ -172,10 +180,11 @@ __device__ __forceinline__ uint64_t to_fixed(float w) {
 }
 
 // Bin of x >= 0 with both ends clamped (gpu_tally.hpp:51-56): x <= 0 and NaN give bin 0, x past
-// the grid the last bin. The clamp is done in float (last = count - 1, exact below 2^24), so the
-// conversion only ever sees values in [0, count - 1].
+// the grid the last bin. The clamp is one v_med3_f32 in float (last = count - 1, exact below
+// 2^24; with a NaN operand med3 returns the minimum of the others, 0), so the conversion only
+// ever sees values in [0, count - 1].
 __device__ __forceinline__ int bin_of(float x, float inv_width, float last) {
-    const float i = __builtin_fminf(__builtin_fmaxf(x * inv_width, 0.0f), last);
+    const float i = __builtin_amdgcn_fmed3f(x * inv_width, 0.0f, last);
     return static_cast<int>(i);
 }
 
@@ -351,24 +360,29 @@ __device__ __forceinline__ void load_stack(const KernelArgs &a, WaveUnit &wu, fl
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
 
-// Add a wave's slot into the HBM rows of its unit and clear it. All 64 lanes take part.
+// Add a wave's slot into the HBM rows of its unit and clear it. All 64 lanes take part. The row
+// pointers are loaded here, through a pointer the compiler cannot see through, so that they are
+// not hoisted into scalar registers that the step loop would have to keep.
 __device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, const WaveUnit &wu,
                                            int lane) {
+    uint64_t *const *cold = a.cold;
+    asm volatile("" : "+s"(cold));
+    uint64_t *rows[kColdRows];
+#pragma unroll
+    for (int k = 0; k < kColdRows; ++k) rows[k] = cold[k];
     const int L = a.lmax;
     for (int j = lane; j < a.slot_words; j += 64) {
         const uint64_t v = slot[j];
         if (v == 0) continue;
         slot[j] = 0;
         uint64_t *dst;
-        if (j == kSlotRd) dst = a.rd + wu.srow;
-        else if (j == kSlotTt) dst = a.tt + wu.srow;
-        else if (j == kSlotEvents) dst = a.events + wu.srow;
-        else if (j == kSlotAnomalies) dst = a.anomalies + wu.srow;
-        else if (j == kSlotBoundary) dst = a.bsteps + wu.srow;
-        else if (j < kSlotAbs + L)
-            dst = a.absorbed + static_cast<uint64_t>(wu.srow) * a.abs_stride + (j - kSlotAbs);
-        else
-            dst = a.az + static_cast<uint64_t>(wu.prow) * a.nz + (j - (kSlotAbs + L));
+        if (j < kSlotAbs) {
+            dst = rows[j] + wu.srow;  // kSlotRd .. kSlotBoundary are kColdRd .. kColdBsteps
+        } else if (j < kSlotAbs + L) {
+            dst = rows[kColdAbsorbed] + static_cast<uint64_t>(wu.srow) * a.abs_stride + (j - kSlotAbs);
+        } else {
+            dst = rows[kColdAz] + static_cast<uint64_t>(wu.prow) * a.nz + (j - (kSlotAbs + L));
+        }
         hbm_add(dst, v);
     }
 }
@@ -911,7 +925,8 @@ size_t arena_bytes(const Problem &p) {
                              (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
                              (p.na > 0 ? 2 * static_cast<size_t>(p.na) * (1 + p.nr) : 0) +
                              2 * static_cast<size_t>(p.nt);
-    return up(U * sizeof(UnitRec)) + up(p.stacks.size() * sizeof(float)) + up(8) + up(out_words * 8);
+    return up(U * sizeof(UnitRec)) + up(p.stacks.size() * sizeof(float)) + up(8 * (1 + kColdRows)) +
+           up(out_words * 8);
 }
 
 // Device memory, kernel arguments and launch shape for one problem shape. A batched plan keeps
@@ -996,7 +1011,7 @@ class Resident {
         n_stacks_floats_ = p.stacks.size();
         units_ = alloc(U * sizeof(UnitRec));
         stacks_ = alloc(p.stacks.size() * sizeof(float));
-        key_ = alloc(8);
+        key_ = alloc(8 * (1 + kColdRows));  // the seed, then the flush rows' pointers
         out_words_ = 1 /*cursor*/ + U /*next*/ + 5 * S + S * p.abs_stride +
                      (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
                      (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
@@ -1039,6 +1054,15 @@ class Resident {
         a_.units = static_cast<const UnitRec *>(units_);
         a_.stacks = static_cast<const float *>(stacks_);
         a_.key = static_cast<const uint64_t *>(key_);
+        a_.cold = reinterpret_cast<uint64_t *const *>(static_cast<uint64_t *>(key_) + 1);
+        key_words_[0] = 0;
+        key_words_[1 + kColdRd] = reinterpret_cast<uintptr_t>(a_.rd);
+        key_words_[1 + kColdTt] = reinterpret_cast<uintptr_t>(a_.tt);
+        key_words_[1 + kColdEvents] = reinterpret_cast<uintptr_t>(a_.events);
+        key_words_[1 + kColdAnomalies] = reinterpret_cast<uintptr_t>(a_.anomalies);
+        key_words_[1 + kColdBsteps] = reinterpret_cast<uintptr_t>(a_.bsteps);
+        key_words_[1 + kColdAbsorbed] = reinterpret_cast<uintptr_t>(a_.absorbed);
+        key_words_[1 + kColdAz] = reinterpret_cast<uintptr_t>(a_.az);
         n_srows_ = S;
         n_prows_ = PR;
 
@@ -1071,7 +1095,8 @@ class Resident {
                                hipMemcpyHostToDevice, stream_));
         HIP_TRY(hipMemcpyAsync(stacks_, p.stacks.data(), p.stacks.size() * sizeof(float),
                                hipMemcpyHostToDevice, stream_));
-        HIP_TRY(hipMemcpyAsync(key_, &seed_, 8, hipMemcpyHostToDevice, stream_));
+        key_words_[0] = seed;
+        HIP_TRY(hipMemcpyAsync(key_, key_words_, sizeof key_words_, hipMemcpyHostToDevice, stream_));
     }
 
     // Clear the outputs and run the transport kernel; returns the kernel time in seconds (HIP
@@ -1199,6 +1224,7 @@ class Resident {
     size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
     void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
     uint64_t seed_ = 0;
+    uint64_t key_words_[1 + kColdRows] = {};  // host copy of the key buffer (must outlive the copy)
     std::vector<void *> allocs_;
     hipStream_t stream_ = nullptr;
     hipEvent_t e0_ = nullptr, e1_ = nullptr;
