@@ -130,8 +130,9 @@ static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
 // The rows a wave's slot is flushed into (ColdRows order): read from device memory when a unit
 // ends, so that the step loop does not keep these pointers in scalar registers.
 enum ColdRow { kColdRd = 0, kColdTt, kColdEvents, kColdAnomalies, kColdBsteps, kColdAbsorbed, kColdAz, kColdRows };
-static_assert(kColdRd == kSlotRd && kColdTt == kSlotTt && kColdEvents == kSlotEvents &&
-                  kColdAnomalies == kSlotAnomalies && kColdBsteps == kSlotBoundary && kColdAbsorbed == kSlotAbs,
+static_assert(int(kColdRd) == int(kSlotRd) && int(kColdTt) == int(kSlotTt) &&
+                  int(kColdEvents) == int(kSlotEvents) && int(kColdAnomalies) == int(kSlotAnomalies) &&
+                  int(kColdBsteps) == int(kSlotBoundary) && int(kColdAbsorbed) == int(kSlotAbs),
               "the slot's scalars and the flush rows share their order");
 
 struct KernelArgs {
@@ -205,6 +206,9 @@ __device__ __forceinline__ void hbm_add(uint64_t *p, uint64_t v) {
 // after that add one by one. Integer sums do not depend on how they are grouped, so the tallies
 // are bit-identical to the plain path's. Every lane of the wave must be active.
 constexpr int kAggPeels = 1;
+
+// Lanes of a wave that wait at a layer boundary before the wave makes their crossings.
+constexpr int kCrossBatch = 6;
 
 // Sum of v over the wave's 64 lanes (every lane active), returned wave-uniform. DPP steps:
 // row_shr 1, 2, 3 of v (zero past the start of a 16-lane row), then row_shr 4 into banks 1-3 and
@@ -429,6 +433,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
     uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
     uint64_t acc = 0;
+    bool pending = false;  // stopped on a layer boundary, its crossing not made yet
     float opl = 0;  // kExit: optical path length so far (cm): sum of n s over the hops
 
     while (have) {
@@ -472,6 +477,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 ev = start ? 0u : ev;
                 bs = start ? 0u : bs;
                 an = start ? 0u : an;
+                pending = start ? false : pending;
                 alive = alive || start;
                 wu.cur += m;
                 need = __ballot(!alive);
@@ -508,17 +514,21 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             const uint32_t r2 = i2 ? (i1 ? q5 : q4) : (i1 ? q3 : q2);
             const uint32_t r3 = i2 ? (i1 ? q6 : q5) : (i1 ? q4 : q3);
 
-            // step_size (physics.hpp:149-158)
+            // step_size (physics.hpp:149-158). A lane waiting at a boundary does not step: its
+            // step length is 0, it draws no words and keeps its remaining step, and it takes no
+            // part in the drop below.
+            const bool go = !pending;
             const bool fresh = sleft == 0.0f;
             float s = fresh ? __builtin_amdgcn_logf(unit_open(r0)) * P.f[kNl2InvMut]
                             : sleft * P.f[kInvMut];
-            uint32_t c = fresh ? 1u : 0u;
-            sleft = 0.0f;
+            uint32_t c = go && fresh ? 1u : 0u;
+            s = go ? s : 0.0f;
+            sleft = go ? 0.0f : sleft;
 
             // hit_boundary (physics.hpp:162-178)
             const float zb = uz > 0.0f ? P.f[kZ1] : P.f[kZ0];
             const float dl = (zb - z) * __builtin_amdgcn_rcpf(uz);
-            const bool hit = (uz != 0.0f) && (s > dl);
+            const bool hit = go && (uz != 0.0f) && (s > dl);
             if (hit) {
                 sleft = (s - dl) * P.f[kMut];
                 s = dl;
@@ -538,89 +548,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             int agg_iz = 0;
             uint32_t agg_ti = 0, agg_bin = 0;
             uint64_t agg_q = 0;
-            if (hit) {
-                // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
-                ++bs;
-                const bool up = uz < 0.0f;
-                const float ni = P.f[kN];
-                const float nt = up ? P.f[kNUp] : P.f[kNDown];
-                const float cc = up ? P.f[kCcTop] : P.f[kCcBot];
-                const float eta = up ? P.f[kEtaUp] : P.f[kEtaDown];
-                const float ca1 = up ? -uz : uz;
-                float r = 1.0f, ca2 = 0.0f;
-                if (ca1 > cc) {
-                    if (ni == nt) {
-                        r = 0.0f;
-                        ca2 = ca1;
-                    } else if (ca1 > 1.0f - 1e-6f) {
-                        r = up ? P.f[kR0Up] : P.f[kR0Down];
-                        ca2 = ca1;
-                    } else if (ca1 < 1e-6f) {
-                        r = 1.0f;
-                    } else {
-                        const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
-                        const float sa2 = sa1 * eta;
-                        if (sa2 >= 1.0f) {
-                            r = 1.0f;
-                        } else {
-                            ca2 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-sa2, sa2, 1.0f));
-                            const float cap = __builtin_fmaf(ca1, ca2, -(sa1 * sa2));
-                            const float cam = __builtin_fmaf(ca1, ca2, sa1 * sa2);
-                            const float sap = __builtin_fmaf(sa1, ca2, ca1 * sa2);
-                            const float sam = __builtin_fmaf(sa1, ca2, -(ca1 * sa2));
-                            r = 0.5f * sam * sam * __builtin_fmaf(cam, cam, cap * cap) *
-                                __builtin_amdgcn_rcpf(sap * sap * cam * cam);
-                        }
-                    }
-                }
-                const float xi = unit_open(c ? r1 : r0);
-                ++c;
-                if (xi > r) {
-                    ux *= eta;
-                    uy *= eta;
-                    uz = up ? -ca2 : ca2;
-                    const bool exit_top = up && layer == 0;
-                    const bool exit_bot = !up && layer == wu.nl - 1;
-                    if (exit_top || exit_bot) {
-                        an += bad_score(ux, uy, uz, w);
-                        if (alive) {
-                            const uint64_t qw = to_fixed(w);
-                            lds_add(slot + (exit_top ? kSlotRd : kSlotTt), qw);
-                            if (kProfiles) {
-                                const float rr =
-                                    __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
-                                const int ir = bin_of(rr, wu.inv_dr, last_r);
-                                uint64_t *row = (exit_top ? a.rr : a.tr) +
-                                                static_cast<uint64_t>(wu.prow) * a.nr;
-                                hbm_add(row + ir, qw);
-                                if (kExit && a.na > 0) {
-                                    // MCML's exit angle: from the normal, acos of the
-                                    // escaping |uz| (= ca2), in na bins of pi / (2 na)
-                                    const int ia = bin_of(acosf(ca2), a.inv_da, last_a);
-                                    lds_add(lds + (exit_top ? 0 : a.na) + ia, qw);
-                                    hbm_add((exit_top ? a.rra : a.tra) +
-                                                __umul24(static_cast<uint32_t>(ir),
-                                                         static_cast<uint32_t>(a.na)) +
-                                                ia,
-                                            qw);
-                                }
-                                if (kExit && a.nt > 0) {
-                                    // time of flight opl / c, in nt bins of dt, clamped
-                                    const int it = bin_of(opl, a.inv_dopl, last_t);
-                                    lds_add(lds + 2 * a.na + (exit_top ? 0 : a.nt) + it, qw);
-                                }
-                            }
-                        }
-                        died = true;
-                    } else {
-                        if (alive && acc != 0) lds_add(slot + kSlotAbs + layer, acc);
-                        acc = 0;
-                        layer += up ? -1 : 1;
-                    }
-                } else {
-                    uz = -uz;
-                }
-            } else {
+            if (!hit && go) {
                 // drop (physics.hpp:188-193) and its score (gpu_tally.hpp:51-56)
                 const float dwa = w * P.f[kAlbedoA];
                 w -= dwa;
@@ -713,8 +641,102 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 aggregate_lds_add(slot + off_az, agg_z, agg_iz, agg_q, lane);
                 if (kArz) aggregate_tile_add(tile, a.arz, agg_t, agg_ti, agg_bin, agg_q, lane);
             }
-            // roulette (physics.hpp:250-259, 274-276)
-            if (!died && w < 1e-4f) {
+            // cross_or_not, deferred: a lane that reached a boundary waits there until
+            // kCrossBatch lanes of the wave wait (or every live lane waits, or the wave drains),
+            // and then they cross together. The crossing branch is divergent code that the whole
+            // wave issues; batching crossings issues it a few times less often. Each photon still
+            // makes the same crossing with the same random word, only later.
+            pending = pending || (hit && alive);
+            const uint64_t waiting = __ballot(pending);
+            if (waiting != 0 &&
+                (__popcll(waiting) >= kCrossBatch || waiting == __ballot(alive) || draining)) {
+                if (pending) {
+                    // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
+                    ++bs;
+                    const bool up = uz < 0.0f;
+                    const float ni = P.f[kN];
+                    const float nt = up ? P.f[kNUp] : P.f[kNDown];
+                    const float cc = up ? P.f[kCcTop] : P.f[kCcBot];
+                    const float eta = up ? P.f[kEtaUp] : P.f[kEtaDown];
+                    const float ca1 = up ? -uz : uz;
+                    float r = 1.0f, ca2 = 0.0f;
+                    if (ca1 > cc) {
+                        if (ni == nt) {
+                            r = 0.0f;
+                            ca2 = ca1;
+                        } else if (ca1 > 1.0f - 1e-6f) {
+                            r = up ? P.f[kR0Up] : P.f[kR0Down];
+                            ca2 = ca1;
+                        } else if (ca1 < 1e-6f) {
+                            r = 1.0f;
+                        } else {
+                            const float sa1 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-ca1, ca1, 1.0f));
+                            const float sa2 = sa1 * eta;
+                            if (sa2 >= 1.0f) {
+                                r = 1.0f;
+                            } else {
+                                ca2 = __builtin_amdgcn_sqrtf(__builtin_fmaf(-sa2, sa2, 1.0f));
+                                const float cap = __builtin_fmaf(ca1, ca2, -(sa1 * sa2));
+                                const float cam = __builtin_fmaf(ca1, ca2, sa1 * sa2);
+                                const float sap = __builtin_fmaf(sa1, ca2, ca1 * sa2);
+                                const float sam = __builtin_fmaf(sa1, ca2, -(ca1 * sa2));
+                                r = 0.5f * sam * sam * __builtin_fmaf(cam, cam, cap * cap) *
+                                    __builtin_amdgcn_rcpf(sap * sap * cam * cam);
+                            }
+                        }
+                    }
+                    const float xi = unit_open(c ? r1 : r0);
+                    ++c;
+                    if (xi > r) {
+                        ux *= eta;
+                        uy *= eta;
+                        uz = up ? -ca2 : ca2;
+                        const bool exit_top = up && layer == 0;
+                        const bool exit_bot = !up && layer == wu.nl - 1;
+                        if (exit_top || exit_bot) {
+                            an += bad_score(ux, uy, uz, w);
+                            if (alive) {
+                                const uint64_t qw = to_fixed(w);
+                                lds_add(slot + (exit_top ? kSlotRd : kSlotTt), qw);
+                                if (kProfiles) {
+                                    const float rr =
+                                        __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
+                                    const int ir = bin_of(rr, wu.inv_dr, last_r);
+                                    uint64_t *row = (exit_top ? a.rr : a.tr) +
+                                                    static_cast<uint64_t>(wu.prow) * a.nr;
+                                    hbm_add(row + ir, qw);
+                                    if (kExit && a.na > 0) {
+                                        // MCML's exit angle: from the normal, acos of the
+                                        // escaping |uz| (= ca2), in na bins of pi / (2 na)
+                                        const int ia = bin_of(acosf(ca2), a.inv_da, last_a);
+                                        lds_add(lds + (exit_top ? 0 : a.na) + ia, qw);
+                                        hbm_add((exit_top ? a.rra : a.tra) +
+                                                    __umul24(static_cast<uint32_t>(ir),
+                                                             static_cast<uint32_t>(a.na)) +
+                                                    ia,
+                                                qw);
+                                    }
+                                    if (kExit && a.nt > 0) {
+                                        // time of flight opl / c, in nt bins of dt, clamped
+                                        const int it = bin_of(opl, a.inv_dopl, last_t);
+                                        lds_add(lds + 2 * a.na + (exit_top ? 0 : a.nt) + it, qw);
+                                    }
+                                }
+                            }
+                            died = true;
+                        } else {
+                            if (alive && acc != 0) lds_add(slot + kSlotAbs + layer, acc);
+                            acc = 0;
+                            layer += up ? -1 : 1;
+                        }
+                    } else {
+                        uz = -uz;
+                    }
+                }
+                pending = false;
+            }
+            // roulette (physics.hpp:250-259, 274-276), after the step's crossing if it had one
+            if (!died && !pending && w < 1e-4f) {
                 if (w == 0.0f) {
                     died = true;
                 } else {
@@ -727,7 +749,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 }
             }
             idx += c;
-            if (++nstep >= a.step_limit && !died) {
+            nstep += go ? 1u : 0u;
+            if (nstep >= a.step_limit && !died && !pending) {
                 ++an;  // a photon this long is reported, not silently dropped
                 died = true;
             }
