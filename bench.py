@@ -181,6 +181,36 @@ def bench_batched(engine, steps: int, warmup: int, tune: bool) -> dict:
             "launch": res[0]["launch"], "autotune": tuned}
 
 
+def bench_small_runs(engine, reps: int = 20) -> dict:
+    """The single-slab workload (1e6 photons, 100 x 100 grid at 0.001 cm, 100 batches, all
+    profiles) and the reference's own benchmark input (semi-infinite tissue, n 1.37, mua 0.1,
+    mus 100, g 0.9; 4e6 photons, 100 batches, 40 x 50 grid; README.md:308 quotes 3.156e9
+    absorption events/s for it on a Tesla T4), each timed over `reps` calls of tmc_hip_run
+    (device state allocated per call) and of tmc_hip_run_static (no allocation). Kernel time
+    from HIP events (tmc_hip_run) and wall time of the whole call."""
+    out = {}
+    for key in ("c1", "tissue"):
+        name, photons, batches, desc = WORKLOADS[key]
+        spec = engine.parse_mci(path=ROOT / "tissuemc_amd" / "configs" / f"{name}.mci")[0]
+        row = {"workload": desc}
+        for label, no_alloc in (("alloc", False), ("static", True)):
+            engine.run(spec, photons=photons, batches=batches, seed=1, no_alloc=no_alloc)  # warm-up
+            t0 = time.perf_counter()
+            rs = [engine.run(spec, photons=photons, batches=batches, seed=1 + i, no_alloc=no_alloc)
+                  for i in range(reps)]
+            wall = (time.perf_counter() - t0) / reps
+            kern = sum(r["kernel_seconds"] for r in rs) / reps
+            events = sum(r["events"] for r in rs) / reps
+            row[label] = {"photons_per_s_wall": photons / wall, "ms_per_run_wall": wall * 1e3,
+                          "kernel_ms": kern * 1e3, "events_per_s_kernel": events / kern,
+                          "events_per_run": events}
+        log(f"bench: {key} {row['alloc']['ms_per_run_wall']:.2f} ms per run "
+            f"({row['static']['ms_per_run_wall']:.2f} ms without allocation), kernel "
+            f"{row['alloc']['kernel_ms']:.2f} ms")
+        out[key] = row
+    return out
+
+
 def bench_surrogate(mc_photons_per_s: float | None = None) -> dict:
     """The surrogate at full size (4 heads, 30 -> 4 x 512 GELU -> 202): live on this GPU, the
     inference kernels on 1.05e6 queries and a burst of training steps (batch 4096; HIP events),
@@ -231,6 +261,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stack workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate kernels")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the single-slab and tissue workloads")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default launch shape instead of autotuning it from pilot runs")
     args = ap.parse_args()
@@ -299,6 +331,8 @@ def main() -> int:
         "launch": results[0]["launch"],
         "autotune": tuned,
     }
+    if not args.no_small:
+        out["small_runs"] = bench_small_runs(engine)
     if not args.no_batched:
         out["batched"] = bench_batched(engine, steps=2, warmup=1, tune=not args.no_tune)
     if not args.no_surrogate:
