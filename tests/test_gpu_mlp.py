@@ -182,3 +182,30 @@ def test_forward_matches_oracle_at_full_width(gpu):
     assert np.allclose(mean, ph.mean(axis=0), rtol=1e-5, atol=1e-6)
     assert np.allclose(spread, ph[:, :, :2].std(axis=0, ddof=1), rtol=1e-3, atol=1e-6)
     assert np.array_equal(mean, mean2) and np.array_equal(spread, spread2)
+
+
+@pytest.mark.parametrize("heads,depth,n", [(4, 4, 40_000), (3, 1, 1000), (1, 2, 129)])
+def test_fused_inference_matches_the_layered_kernels(gpu, monkeypatch, heads, depth, n):
+    """Hidden width 512 with 30 inputs and 202 outputs goes through the fused whole-network
+    kernel; TMC_MLP_INFER=layered runs the layer-by-layer products. Same MFMA, same operand
+    order, same bias seeding and GELU: the outputs agree bit for bit. Row counts cross a chunk
+    (one workgroup of 128 rows per CU) and end mid-tile."""
+    from tissuemc_amd.mlp import Mlp
+
+    cfg = dict(heads=heads, inputs=30, hidden=512, depth=depth, outputs=202, batch=1024, seed=5)
+    rng = np.random.default_rng(heads * 10 + depth)
+    x = rng.normal(0, 1, (n, 30)).astype(np.float32)
+    with Mlp(**cfg) as fused:
+        # trained a few steps, so the weights are not the initial draw
+        fused.set_data(rng.normal(0, 1, (1024, 30)).astype(np.float32),
+                       rng.normal(0, 1, (1024, 202)).astype(np.float32))
+        fused.train(3)
+        params = fused.get_params()
+        mean_f, spread_f, ph_f, _ = fused.predict(x, per_head=True)
+    monkeypatch.setenv("TMC_MLP_INFER", "layered")
+    with Mlp(**cfg) as layered:
+        layered.set_params(params)
+        mean_l, spread_l, ph_l, _ = layered.predict(x, per_head=True)
+    assert np.array_equal(ph_f, ph_l)
+    assert np.array_equal(mean_f, mean_l) and np.array_equal(spread_f, spread_l)
+    assert np.all(np.isfinite(ph_f)) and np.abs(ph_f).max() > 0

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -964,9 +965,8 @@ __global__ void load_inputs(const float *xf, int rows, int in, int in_used, int 
     *reinterpret_cast<uint4 *>(X + static_cast<long long>(m) * in_pad + j0) = uint4{w[0], w[1], w[2], w[3]};
 }
 
-// Ensemble mean of every output and the spread (standard deviation over heads) of the first
-// n_spread outputs, for rows [0, rows).
-// Ensemble mean of every output and spread of the first n_spread (<= 4), per query row m < rows:
+// Ensemble mean of every output and spread (standard deviation over heads) of the first
+// n_spread (<= 4), per query row m < rows:
 // four rows per block, 64 threads per row, four consecutive outputs per thread (16-byte loads;
 // out_pad is a multiple of 4 at least out rounded up to 4).
 __global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int heads, int B, int out_pad,
@@ -995,6 +995,164 @@ __global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int head
                     spread[static_cast<long long>(m) * n_spread + c + q] =
                         heads > 1 ? __builtin_sqrtf(ss[q] / (heads - 1)) : 0.0f;
         }
+    }
+}
+
+// ---- fused ensemble inference
+//
+// The whole network of every head for 128 query rows in one workgroup of eight waves (one
+// workgroup per CU): the activations stay in LDS from layer to layer, a 128 x 512 bf16 tile, and
+// only the inputs come in and each head's output layer goes out. Each wave computes 128 rows x
+// (1/8 of the layer's columns) with the same MFMA as the layer-by-layer kernels (swapped
+// operands, bias-seeded accumulators, K in steps of 32 in order), so its results are the same bit
+// for bit. Its weight fragments come straight from global memory (L2) into registers, one K
+// step ahead, from a copy of the weights in fragment order (pack_fragments): a wave's fragment
+// of 16 output rows x 32 K is 1 KB of consecutive bytes, whole cache lines, where the [out x in]
+// layout would give it sixteen half lines (measured: 24% -> 32% of the MFMA peak). A lane's 16
+// bytes are the MFMA fragment itself, so the K loop has no LDS staging and no barrier. The activation fragments are read
+// from LDS (rows padded by 16 bytes: the 16 rows of a fragment fall in distinct bank groups).
+// Between layers the waves meet twice: before the GELU results overwrite the tile every wave has
+// read, and after, before the next layer reads them.
+constexpr int kFusedRows = 128;
+constexpr int kFusedHid = 512;               // hidden width the kernel is built for
+constexpr int kFusedOut = 256;               // padded output width
+constexpr int kFusedK0 = 32;                 // K of the first layer (the inputs, padded)
+constexpr int kFusedPitch = kFusedHid + 8;   // bf16 pitch of the activation tile
+constexpr int kFusedInPitch = kFusedK0 + 8;  // bf16 pitch of the input tile
+constexpr int kFusedLds = kFusedRows * (kFusedPitch + kFusedInPitch) * 2;  // bytes
+
+// Weights in fragment order, per matrix [out x in] with the first `kuse` columns multiplied:
+// element (n, k) at ((n / 16) (kuse / 32) + k / 32) 512 + lane 8 + k % 8, lane = 16 (k / 8 % 4)
+// + n % 16. The fragment a lane loads for a 16-row block and a K step is its 8 elements there,
+// and the wave's 64 lanes read 1 KB of consecutive bytes.
+struct FragArgs {
+    const bf16 *wb;  // [out x in] copies (head stride sWb), matrix m at wt[m]
+    long long sWb;
+    bf16 *wf;  // fragment-order copies (head stride sWf), matrix m at fo[m]
+    long long sWf;
+    long long wt[kMaxMats], fo[kMaxMats];
+    int out[kMaxMats], in[kMaxMats], kuse[kMaxMats];
+};
+
+// One thread per 16-byte fragment piece; blockIdx.y = matrix, blockIdx.z = head.
+__global__ __launch_bounds__(256) void pack_fragments(const FragArgs f) {
+    const int m = static_cast<int>(blockIdx.y), h = static_cast<int>(blockIdx.z);
+    const long long pieces = static_cast<long long>(f.out[m]) * (f.kuse[m] / 8);
+    const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= pieces) return;
+    const int lane = static_cast<int>(c & 63);
+    const long long blk = c >> 6;  // (n / 16) (kuse / 32) + k / 32
+    const int kts = f.kuse[m] / kBK;
+    const int n = static_cast<int>(blk / kts) * 16 + (lane & 15);
+    const int k = static_cast<int>(blk % kts) * kBK + (lane >> 4) * 8;
+    *reinterpret_cast<uint4 *>(f.wf + h * f.sWf + f.fo[m] + c * 8) =
+        *reinterpret_cast<const uint4 *>(f.wb + h * f.sWb + f.wt[m] + static_cast<long long>(n) * f.in[m] + k);
+}
+
+struct FusedArgs {
+    const bf16 *X;  // network inputs [rows x in_pad] (the first kFusedK0 columns are used)
+    int in_pad;
+    const bf16 *W;  // fragment-order weights of head 0 (pack_fragments); head h at W + h sW,
+    long long sW;   // layer l at wt[l]
+    const float *bias;  // biases of head 0 (head stride sBias), layer l's at bo[l]
+    long long sBias;
+    long long wt[kMaxMats], bo[kMaxMats];
+    int depth, heads;
+    float *Yp;  // [heads x rows x kFusedOut] fp32 outputs, head stride sYp
+    long long sYp;
+};
+
+// One layer's products for the wave's columns n0 .. n0 + 16 JF - 1 and all 128 rows: acc =
+// bias + A . W^T with A [128 x 32 NK] in LDS (row pitch PITCH) and W [N x 32 NK] in global
+// memory in fragment order.
+// The K loop is unrolled into groups of one activation fragment (16 rows x 32 K) and its JF
+// MFMAs, issued in this order: the fragment for three groups ahead is read from LDS and, at the
+// first group of a K step, the weights of the next step are loaded, then the group's MFMAs. The
+// schedule barrier keeps the compiler from sinking the loads next to their use, where their
+// latency would stall the wave.
+template <int JF, int NK, int PITCH>
+__device__ __forceinline__ void fused_layer(const bf16 *As, const bf16 *W, int n0, const float *bias,
+                                            floatx4 (&acc)[8][JF], int lane) {
+    constexpr int kAhead = 3, kRing = 4, kGroups = 8 * NK;
+    const int r = lane & 15, kq = lane >> 4, g4 = kq * 4;
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+        const floatx4 init = *reinterpret_cast<const floatx4 *>(bias + n0 + 16 * j + g4);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i][j] = init;
+    }
+    const bf16 *wl[JF];
+#pragma unroll
+    for (int j = 0; j < JF; ++j) wl[j] = W + (static_cast<long long>((n0 / 16 + j) * NK) * 64 + lane) * 8;
+    const bf16 *al = As + r * PITCH + kq * 8;
+    auto a_frag = [&](int g) {
+        return *reinterpret_cast<const bf16x8 *>(al + 16 * (g & 7) * PITCH + (g >> 3) * kBK);
+    };
+    bf16x8 b[2][JF], fa[kRing];
+#pragma unroll
+    for (int j = 0; j < JF; ++j) b[0][j] = *reinterpret_cast<const bf16x8 *>(wl[j]);
+#pragma unroll
+    for (int g = 0; g < kAhead && g < kGroups; ++g) fa[g % kRing] = a_frag(g);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        const int kt = g >> 3, i = g & 7;
+        if (i == 0 && kt + 1 < NK) {
+#pragma unroll
+            for (int j = 0; j < JF; ++j)
+                b[(kt + 1) & 1][j] = *reinterpret_cast<const bf16x8 *>(wl[j] + (kt + 1) * 512);
+        }
+        if (g + kAhead < kGroups) fa[(g + kAhead) % kRing] = a_frag(g + kAhead);
+#pragma unroll
+        for (int j = 0; j < JF; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[kt & 1][j], fa[g % kRing], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void infer_fused(const FusedArgs a) {
+    extern __shared__ __attribute__((aligned(16))) bf16 smem[];
+    bf16 *Xs = smem;                                 // activations [128 x kFusedPitch]
+    bf16 *Ins = smem + kFusedRows * kFusedPitch;     // inputs [128 x kFusedInPitch]
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g4 = (lane >> 4) * 4;
+    const long long row0 = static_cast<long long>(blockIdx.x) * kFusedRows;
+    {
+        const int rr = tid >> 2, c = (tid & 3) * 8;  // 512 threads: 128 rows x 4 pieces of 16 bytes
+        *reinterpret_cast<uint4 *>(Ins + rr * kFusedInPitch + c) =
+            *reinterpret_cast<const uint4 *>(a.X + (row0 + rr) * a.in_pad + c);
+    }
+    __syncthreads();
+    for (int h = 0; h < a.heads; ++h) {
+        const bf16 *W = a.W + h * a.sW;
+        const float *B = a.bias + h * a.sBias;
+        for (int l = 0; l < a.depth; ++l) {
+            floatx4 acc[8][4];
+            const int n0 = wave * (kFusedHid / 8);
+            if (l == 0)
+                fused_layer<4, kFusedK0 / kBK, kFusedInPitch>(Ins, W + a.wt[0], n0, B + a.bo[0], acc, lane);
+            else
+                fused_layer<4, kFusedHid / kBK, kFusedPitch>(Xs, W + a.wt[l], n0, B + a.bo[l], acc, lane);
+            __syncthreads();  // every wave has read the tile (and the previous head's last layer)
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const floatx4 &zz = acc[i][j];
+                    *reinterpret_cast<uint2 *>(Xs + (16 * i + r) * kFusedPitch + n0 + 16 * j + g4) =
+                        uint2{pack_bf16x2(gelu(zz[0]), gelu(zz[1])), pack_bf16x2(gelu(zz[2]), gelu(zz[3]))};
+                }
+            __syncthreads();
+        }
+        floatx4 acc[8][2];
+        const int n0 = wave * (kFusedOut / 8);
+        fused_layer<2, kFusedHid / kBK, kFusedPitch>(Xs, W + a.wt[a.depth], n0, B + a.bo[a.depth], acc, lane);
+        float *Y = a.Yp + h * a.sYp;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                *reinterpret_cast<floatx4 *>(Y + (row0 + 16 * i + r) * kFusedOut + n0 + 16 * j + g4) = acc[i][j];
     }
 }
 
@@ -1149,7 +1307,20 @@ class MlpModel {
         out_pad_ = pad128(c.outputs);
         hid_ = c.hidden;
         B_ = c.batch;
-        Bq_ = std::max(B_, 16384);
+        // The fused inference kernel (one workgroup of 128 rows per CU, activations in LDS) serves
+        // networks of hidden width 512 with at most 32 inputs and 129..256 outputs; a chunk of
+        // queries is one workgroup per CU. TMC_MLP_INFER=layered picks the layer-by-layer kernels.
+        {
+            const char *pick = std::getenv("TMC_MLP_INFER");
+            const bool layered = pick && std::strcmp(pick, "layered") == 0;
+            fused_ = !layered && hid_ == mlp::kFusedHid && k0_ == mlp::kFusedK0 && out_pad_ == mlp::kFusedOut;
+        }
+        int cus = 0;
+        MLP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        Bq_ = std::max(B_, fused_ ? std::max(cus, 1) * mlp::kFusedRows : 16384);
+        if (fused_)
+            MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::infer_fused),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kFusedLds));
         // parameter layout per head: W [out x in] then b [out], per matrix
         long long off = 0, boff = 0;
         nb_ = 0;
@@ -1211,6 +1382,13 @@ class MlpModel {
         Hq_[0] = alloc<bf16>(H_ * QH);
         Hq_[1] = alloc<bf16>(H_ * QH);
         xq_ = alloc<float>(static_cast<long long>(Bq_) * c.inputs);
+        if (fused_) {
+            for (size_t m = 0; m < mats_.size(); ++m) {
+                frag_off_.push_back(Pf_);
+                Pf_ += static_cast<long long>(mats_[m].out) * (m == 0 ? k0_ : mats_[m].in);
+            }
+            wf_ = alloc<bf16>(H_ * Pf_);
+        }
         mean_ = alloc<float>(static_cast<long long>(Bq_) * c.outputs);
         spread_ = alloc<float>(static_cast<long long>(Bq_) * 2);
         init_params();
@@ -1280,6 +1458,7 @@ class MlpModel {
             MLP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
         }
         MLP_TRY(hipEventRecord(e0_, stream_));
+        if (steps > 0) frag_stale_ = true;
         for (int64_t s = 0; s < steps; ++s) MLP_TRY(hipGraphLaunch(graph_exec_, stream_));
         MLP_TRY(hipEventRecord(e1_, stream_));
         MLP_TRY(hipEventSynchronize(e1_));
@@ -1332,7 +1511,7 @@ class MlpModel {
             MLP_TRY(hipEventRecord(e0_, stream_));
             hipLaunchKernelGGL(mlp::load_inputs, dim3((Bq_ * (k0_ / 8) + 255) / 256), dim3(256), 0,
                                stream_, xq_, rows, cfg_.inputs, k0_, in_pad_, Bq_, Xq_);
-            forward(Xq_, Bq_, Hq_, nullptr, true);
+            infer(rows);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             MLP_TRY(hipGetLastError());
@@ -1439,7 +1618,7 @@ class MlpModel {
             ea.X = Xq_;
             ea.dom = ans_dom_;
             hipLaunchKernelGGL(mlp::encode_stacks, dim3((Bq_ + 255) / 256), dim3(256), 0, stream_, ea);
-            forward(Xq_, Bq_, Hq_, nullptr, true);
+            infer(rows);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             mlp::FinishArgs fa{mean_, spread_, ans_dom_, ans_ymean_, ans_ystd_, rows, cfg_.outputs,
@@ -1552,6 +1731,7 @@ class MlpModel {
     }
 
     void refresh_bf16() {
+        frag_stale_ = true;
         mlp::AdamArgs a;
         std::memset(&a, 0, sizeof a);
         a.p = p_;
@@ -1622,6 +1802,57 @@ class MlpModel {
 
     // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
     // alternate between two buffers and neither Z nor Ht is kept.
+    // Every head's outputs for the first `rows` queries of Xq_ into Yp_ [head x Bq_ x out_pad_]:
+    // the fused kernel where the network's shape allows it, else the layer-by-layer products
+    // over the whole chunk.
+    void infer(int rows) {
+        if (!fused_) {
+            forward(Xq_, Bq_, Hq_, nullptr, true);
+            return;
+        }
+        if (frag_stale_) {
+            mlp::FragArgs f;
+            std::memset(&f, 0, sizeof f);
+            f.wb = wb_;
+            f.sWb = Pb_;
+            f.wf = wf_;
+            f.sWf = Pf_;
+            long long most = 0;
+            for (size_t m = 0; m < mats_.size(); ++m) {
+                f.wt[m] = mats_[m].wt_off;
+                f.fo[m] = frag_off_[m];
+                f.out[m] = mats_[m].out;
+                f.in[m] = mats_[m].in;
+                f.kuse[m] = m == 0 ? k0_ : mats_[m].in;
+                most = std::max(most, static_cast<long long>(f.out[m]) * (f.kuse[m] / 8));
+            }
+            hipLaunchKernelGGL(mlp::pack_fragments, dim3(static_cast<unsigned>((most + 255) / 256),
+                               static_cast<unsigned>(mats_.size()), H_), dim3(256), 0, stream_, f);
+            MLP_TRY(hipGetLastError());
+            frag_stale_ = false;
+        }
+        mlp::FusedArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.X = Xq_;
+        a.in_pad = in_pad_;
+        a.W = wf_;
+        a.sW = Pf_;
+        a.bias = p_;
+        a.sBias = P_;
+        for (int l = 0; l <= cfg_.depth; ++l) {
+            a.wt[l] = frag_off_[l];
+            a.bo[l] = mats_[l].boff;
+        }
+        a.depth = cfg_.depth;
+        a.heads = H_;
+        a.Yp = Yp_;
+        a.sYp = static_cast<long long>(Bq_) * out_pad_;
+        const int tiles = (rows + mlp::kFusedRows - 1) / mlp::kFusedRows;
+        if (tiles * mlp::kFusedRows > Bq_) throw EngineError(TMC_ERR_RUNTIME, "fused inference: rows exceed the chunk");
+        hipLaunchKernelGGL(mlp::infer_fused, dim3(tiles), dim3(512), mlp::kFusedLds, stream_, a);
+        MLP_TRY(hipGetLastError());
+    }
+
     void forward(bf16 *X, int Bn, bf16 *const *Hbuf, bf16 *const *Gbuf, bool infer) {
         const long long BH = static_cast<long long>(Bn) * hid_;
         const bf16 *in = X;
@@ -1788,6 +2019,13 @@ class MlpModel {
 
     tmc_mlp_config cfg_;
     int H_ = 1, in_pad_ = 128, out_pad_ = 128, hid_ = 128, B_ = 128, Bq_ = 128;
+    bool fused_ = false;  // inference through infer_fused
+    // its weights in fragment order (pack_fragments), per head Pf_ elements, matrix m at
+    // frag_off_[m]; stale after every parameter change until the next inference repacks them
+    bf16 *wf_ = nullptr;
+    long long Pf_ = 0;
+    std::vector<long long> frag_off_;
+    bool frag_stale_ = true;
     int k0_ = 128;  // K of the first layer: the inputs rounded up to a stage (32)
     long long P_ = 0, Pb_ = 0;
     std::vector<mlp::MatInfo> mats_;
