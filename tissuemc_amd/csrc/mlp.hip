@@ -1800,8 +1800,6 @@ class MlpModel {
         return g;
     }
 
-    // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
-    // alternate between two buffers and neither Z nor Ht is kept.
     // Every head's outputs for the first `rows` queries of Xq_ into Yp_ [head x Bq_ x out_pad_]:
     // the fused kernel where the network's shape allows it, else the layer-by-layer products
     // over the whole chunk.
@@ -1853,6 +1851,8 @@ class MlpModel {
         MLP_TRY(hipGetLastError());
     }
 
+    // Forward through every layer: X [Bn x in_pad] per head -> Yp_. With `infer`, activations
+    // alternate between two buffers and neither Z nor Ht is kept.
     void forward(bf16 *X, int Bn, bf16 *const *Hbuf, bf16 *const *Gbuf, bool infer) {
         const long long BH = static_cast<long long>(Bn) * hid_;
         const bf16 *in = X;
