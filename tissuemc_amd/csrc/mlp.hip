@@ -26,8 +26,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device_rng.h"
@@ -1273,6 +1278,62 @@ __global__ __launch_bounds__(256) void finish_profiles(const float *mean, const 
 
 using mlp::bf16;
 
+// A few host threads that split a loop between them, the calling thread taking a share: the
+// packing of a chunk of stacks, which reads the caller's stack array (memory-bound, a few hundred
+// bytes a stack) faster from several cores than from one.
+class HostTeam {
+  public:
+    explicit HostTeam(int n) : size_(std::max(n, 1)) {
+        for (int t = 1; t < size_; ++t) workers_.emplace_back([this, t] { serve(t); });
+    }
+    ~HostTeam() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            quit_ = true;
+            ++gen_;
+        }
+        go_.notify_all();
+        for (auto &w : workers_) w.join();
+    }
+    // f(begin, end) over [0, rows) in size_ contiguous pieces; returns when all are done
+    void run(int64_t rows, const std::function<void(int64_t, int64_t)> &f) {
+        job_ = [&](int t) { f(rows * t / size_, rows * (t + 1) / size_); };
+        {
+            std::lock_guard<std::mutex> g(m_);
+            pending_ = size_ - 1;
+            ++gen_;
+        }
+        go_.notify_all();
+        job_(0);
+        std::unique_lock<std::mutex> l(m_);
+        done_.wait(l, [&] { return pending_ == 0; });
+    }
+
+  private:
+    void serve(int t) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                go_.wait(l, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (quit_) return;
+            }
+            job_(t);
+            std::lock_guard<std::mutex> g(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    int size_;
+    std::vector<std::thread> workers_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    std::function<void(int)> job_;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    bool quit_ = false;
+};
+
 class MlpModel {
   public:
     static int pad128(int x) { return (x + 127) / 128 * 128; }
@@ -1409,7 +1470,10 @@ class MlpModel {
             if (ans_k0_[i]) (void)hipEventDestroy(ans_k0_[i]);
             if (ans_k1_[i]) (void)hipEventDestroy(ans_k1_[i]);
             if (ans_done_[i]) (void)hipEventDestroy(ans_done_[i]);
+            if (ans_h2d_[i]) (void)hipEventDestroy(ans_h2d_[i]);
+            if (ans_used_[i]) (void)hipEventDestroy(ans_used_[i]);
         }
+        if (ans_copy_) (void)hipStreamDestroy(ans_copy_);
         if (stream_) (void)hipStreamDestroy(stream_);
         for (void *q : allocs_) (void)hipFree(q);
         for (void *q : pinned_) (void)hipHostFree(q);
@@ -1536,8 +1600,9 @@ class MlpModel {
     }
 
     // Gated surrogate answers for stacks, end to end (tmc_mlp_answer): chunks of Bq_ stacks,
-    // the host packing chunk k + 1 into pinned memory while the device encodes, runs and gates
-    // chunk k; two pinned slots alternate.
+    // the host packing chunk k + 1 into pinned memory (a few threads) while its copy to the
+    // device runs on a stream of its own and the device encodes, runs and gates chunk k; two
+    // pinned slots and two device input buffers alternate.
     void answer(const tmc_surrogate_gate &gate, const tmc_stack *stacks, int64_t n, tmc_surrogate_answers *out) {
         const auto t_start = std::chrono::steady_clock::now();
         if (cfg_.inputs != mlp::kAnswerInputs)
@@ -1589,35 +1654,49 @@ class MlpModel {
             if (want_prof)
                 std::memcpy(out->profiles + r0 * width, ans_pin_prof_[slot], static_cast<size_t>(rows) * width * 4);
         };
+        // packing threads: only for more than one chunk's worth, at most 8 (the copy is memory-bound)
+        const unsigned hw = std::thread::hardware_concurrency();
+        std::unique_ptr<HostTeam> team;
+        if (n > Bq_ / 2) team = std::make_unique<HostTeam>(static_cast<int>(std::clamp(hw, 1u, 8u)));
         for (int64_t k = 0; k < chunks; ++k) {
             const int slot = static_cast<int>(k & 1), rows = rows_of(k);
             if (k >= 2) drain(k - 2);
             // pack: layer count, then the five parameters of the first kSlots layers (0 if absent)
             double *raw = ans_pin_raw_[slot];
-            for (int m = 0; m < rows; ++m) {
-                const tmc_stack &st = stacks[k * Bq_ + m];
-                double *r = raw + static_cast<long long>(m) * mlp::kRawPerStack;
-                r[0] = static_cast<double>(st.num_layers);
-                for (int i = 0; i < mlp::kSlots; ++i) {
-                    double *q = r + 1 + 5 * i;
-                    if (i < st.num_layers && i < TMC_MAX_LAYERS) {
-                        const tmc_layer &L = st.layers[i];
-                        q[0] = L.mua; q[1] = L.mus; q[2] = L.g; q[3] = L.n; q[4] = L.d;
-                    } else {
-                        q[0] = q[1] = q[2] = q[3] = q[4] = 0.0;
+            const tmc_stack *first = stacks + k * Bq_;
+            auto pack = [&](int64_t m0, int64_t m1) {
+                for (int64_t m = m0; m < m1; ++m) {
+                    const tmc_stack &st = first[m];
+                    double *r = raw + m * mlp::kRawPerStack;
+                    r[0] = static_cast<double>(st.num_layers);
+                    for (int i = 0; i < mlp::kSlots; ++i) {
+                        double *q = r + 1 + 5 * i;
+                        if (i < st.num_layers && i < TMC_MAX_LAYERS) {
+                            const tmc_layer &L = st.layers[i];
+                            q[0] = L.mua; q[1] = L.mus; q[2] = L.g; q[3] = L.n; q[4] = L.d;
+                        } else {
+                            q[0] = q[1] = q[2] = q[3] = q[4] = 0.0;
+                        }
                     }
                 }
-            }
-            MLP_TRY(hipMemcpyAsync(ans_raw_, raw, static_cast<size_t>(rows) * mlp::kRawPerStack * 8,
-                                   hipMemcpyHostToDevice, stream_));
+            };
+            if (team) team->run(rows, pack);
+            else pack(0, rows);
+            // the copy waits until the encoding of chunk k - 2 has read this device buffer
+            MLP_TRY(hipStreamWaitEvent(ans_copy_, ans_used_[slot], 0));
+            MLP_TRY(hipMemcpyAsync(ans_raw_[slot], raw, static_cast<size_t>(rows) * mlp::kRawPerStack * 8,
+                                   hipMemcpyHostToDevice, ans_copy_));
+            MLP_TRY(hipEventRecord(ans_h2d_[slot], ans_copy_));
+            MLP_TRY(hipStreamWaitEvent(stream_, ans_h2d_[slot], 0));
             MLP_TRY(hipEventRecord(ans_k0_[slot], stream_));
-            ea.raw = ans_raw_;
+            ea.raw = ans_raw_[slot];
             ea.rows = rows;
             ea.Bq = Bq_;
             ea.in_pad = in_pad_;
             ea.X = Xq_;
             ea.dom = ans_dom_;
             hipLaunchKernelGGL(mlp::encode_stacks, dim3((Bq_ + 255) / 256), dim3(256), 0, stream_, ea);
+            MLP_TRY(hipEventRecord(ans_used_[slot], stream_));
             infer(rows);
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
@@ -2032,7 +2111,9 @@ class MlpModel {
     std::vector<int> real_in_, real_out_;
     std::vector<void *> allocs_;
     // surrogate answers (allocated on first use): device buffers, pinned slots, events
-    double *ans_raw_ = nullptr, *ans_res_ = nullptr;
+    double *ans_raw_[2] = {nullptr, nullptr}, *ans_res_ = nullptr;
+    hipStream_t ans_copy_ = nullptr;  // the answers' input copies
+    hipEvent_t ans_h2d_[2] = {nullptr, nullptr}, ans_used_[2] = {nullptr, nullptr};
     uint8_t *ans_dom_ = nullptr, *ans_route_ = nullptr;
     float *ans_prof_ = nullptr, *ans_ymean_ = nullptr, *ans_ystd_ = nullptr;
     double *ans_pin_raw_[2] = {nullptr, nullptr}, *ans_pin_res_[2] = {nullptr, nullptr};
@@ -2041,14 +2122,17 @@ class MlpModel {
     std::vector<void *> pinned_;
 
     void answer_buffers(bool profiles) {
-        if (!ans_raw_) {
-            ans_raw_ = alloc<double>(static_cast<long long>(Bq_) * mlp::kRawPerStack);
+        if (!ans_res_) {
+            MLP_TRY(hipStreamCreateWithFlags(&ans_copy_, hipStreamNonBlocking));
             ans_res_ = alloc<double>(4ll * Bq_);
             ans_dom_ = alloc<uint8_t>(Bq_);
             ans_route_ = alloc<uint8_t>(Bq_);
             ans_ymean_ = alloc<float>(cfg_.outputs);
             ans_ystd_ = alloc<float>(cfg_.outputs);
             for (int i = 0; i < 2; ++i) {
+                ans_raw_[i] = alloc<double>(static_cast<long long>(Bq_) * mlp::kRawPerStack);
+                MLP_TRY(hipEventCreateWithFlags(&ans_h2d_[i], hipEventDisableTiming));
+                MLP_TRY(hipEventCreateWithFlags(&ans_used_[i], hipEventDisableTiming));
                 ans_pin_raw_[i] = pinned<double>(static_cast<size_t>(Bq_) * mlp::kRawPerStack);
                 ans_pin_res_[i] = pinned<double>(4 * static_cast<size_t>(Bq_) + (2 * static_cast<size_t>(Bq_) + 7) / 8);
                 MLP_TRY(hipEventCreate(&ans_k0_[i]));
