@@ -1007,10 +1007,10 @@ __global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int head
 //
 // The whole network of every head for 128 query rows in one workgroup of eight waves (one
 // workgroup per CU): the activations stay in LDS from layer to layer, a 128 x 512 bf16 tile, and
-// only the inputs come in and each head's output layer goes out. Each wave computes 128 rows x
-// (1/8 of the layer's columns) with the same MFMA as the layer-by-layer kernels (swapped
-// operands, bias-seeded accumulators, K in steps of 32 in order), so its results are the same bit
-// for bit. Its weight fragments come straight from global memory (L2) into registers, one K
+// only the inputs come in and each head's output layer goes out. Each wave computes 1/8 of a
+// layer's columns for all 128 rows, one half of the rows at a time, with the same MFMA as the
+// layer-by-layer kernels (swapped operands, bias-seeded accumulators, K in steps of 32 in
+// order), so its results are the same bit for bit. Its weight fragments come straight from global memory (L2) into registers, one K
 // step ahead, from a copy of the weights in fragment order (pack_fragments): a wave's fragment
 // of 16 output rows x 32 K is 1 KB of consecutive bytes, whole cache lines, where the [out x in]
 // layout would give it sixteen half lines (measured: 24% -> 32% of the MFMA peak). A lane's 16
@@ -1067,40 +1067,42 @@ struct FusedArgs {
     long long sYp;
 };
 
-// One layer's products for the wave's columns n0 .. n0 + 16 JF - 1 and all 128 rows: acc =
-// bias + A . W^T with A [128 x 32 NK] in LDS (row pitch PITCH) and W [N x 32 NK] in global
-// memory in fragment order.
-// The K loop is unrolled into groups of one activation fragment (16 rows x 32 K) and its JF
-// MFMAs, issued in this order: the fragment for three groups ahead is read from LDS and, at the
-// first group of a K step, the weights of the next step are loaded, then the group's MFMAs. The
-// schedule barrier keeps the compiler from sinking the loads next to their use, where their
-// latency would stall the wave.
-template <int JF, int NK, int PITCH>
-__device__ __forceinline__ void fused_layer(const bf16 *As, const bf16 *W, int n0, const float *bias,
-                                            floatx4 (&acc)[8][JF], int lane) {
-    constexpr int kAhead = 3, kRing = 4, kGroups = 8 * NK;
+// One layer's products for the wave's columns n0 .. n0 + 16 JF - 1 and one half of the rows
+// (64): acc = bias + A . W^T with A [64 x 32 NK] in LDS (row pitch PITCH) and W [N x 32 NK] in
+// global memory in fragment order. The K loop is unrolled into groups of one activation fragment
+// (16 rows x 32 K) and its JF MFMAs, issued in this order: the fragment for three groups ahead
+// is read from LDS and, at the first group of a K step, the weights of the next step are loaded,
+// then the group's MFMAs, then a share of `pend`'s UNITS units of other work (the other half's
+// epilogue), spread evenly over the groups so that its VALU work issues while the MFMAs run. The
+// schedule barrier after each group keeps the compiler from sinking the loads next to their use,
+// where their latency would stall the wave, or from bunching the epilogue work.
+template <int JF, int NK, int PITCH, int UNITS, class Pend>
+__device__ __forceinline__ void fused_half(const bf16 *As, const bf16 *W, int n0, const float *bias,
+                                           floatx4 (&acc)[4][JF], int lane, const Pend &pend) {
+    constexpr int kAhead = 3, kRing = 4, kGroups = 4 * NK;
     const int r = lane & 15, kq = lane >> 4, g4 = kq * 4;
 #pragma unroll
     for (int j = 0; j < JF; ++j) {
         const floatx4 init = *reinterpret_cast<const floatx4 *>(bias + n0 + 16 * j + g4);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) acc[i][j] = init;
+        for (int i = 0; i < 4; ++i) acc[i][j] = init;
     }
     const bf16 *wl[JF];
 #pragma unroll
     for (int j = 0; j < JF; ++j) wl[j] = W + (static_cast<long long>((n0 / 16 + j) * NK) * 64 + lane) * 8;
     const bf16 *al = As + r * PITCH + kq * 8;
     auto a_frag = [&](int g) {
-        return *reinterpret_cast<const bf16x8 *>(al + 16 * (g & 7) * PITCH + (g >> 3) * kBK);
+        return *reinterpret_cast<const bf16x8 *>(al + 16 * (g & 3) * PITCH + (g >> 2) * kBK);
     };
     bf16x8 b[2][JF], fa[kRing];
 #pragma unroll
     for (int j = 0; j < JF; ++j) b[0][j] = *reinterpret_cast<const bf16x8 *>(wl[j]);
 #pragma unroll
     for (int g = 0; g < kAhead && g < kGroups; ++g) fa[g % kRing] = a_frag(g);
-#pragma unroll
+    static_assert(UNITS == 0 || UNITS % kGroups == 0 || kGroups % UNITS == 0, "units spread evenly");
+#pragma clang loop unroll(full)
     for (int g = 0; g < kGroups; ++g) {
-        const int kt = g >> 3, i = g & 7;
+        const int kt = g >> 2, i = g & 3;
         if (i == 0 && kt + 1 < NK) {
 #pragma unroll
             for (int j = 0; j < JF; ++j)
@@ -1110,10 +1112,51 @@ __device__ __forceinline__ void fused_layer(const bf16 *As, const bf16 *W, int n
 #pragma unroll
         for (int j = 0; j < JF; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[kt & 1][j], fa[g % kRing], acc[i][j], 0, 0, 0);
+        if constexpr (UNITS >= kGroups) {
+#pragma unroll
+            for (int u = 0; u < UNITS / kGroups; ++u) pend(g * (UNITS / kGroups) + u);
+        } else if constexpr (UNITS > 0) {
+            if (g % (kGroups / UNITS) == kGroups / UNITS - 1) pend(g / (kGroups / UNITS));
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
+// Epilogue units for fused_half's `pend`. GeluPending: unit v of 64 is the GELU of value v % 4 of
+// the lane's four in 16 x 16 tile t = v / 4 of a hidden layer's half (i = t / 4, j = t % 4); at
+// the fourth the tile's bf16 values are stored into the activation tile at `dst` (the lane's row
+// and first column of that half), so that a group's MFMAs share the issue slots with one GELU.
+// OutputPending: unit u of 8 is the fp32 store of one output tile.
+struct NoPending {
+    __device__ void operator()(int) const {}
+};
+struct GeluPending {
+    const floatx4 (&acc)[4][4];
+    bf16 *dst;
+    mutable float done[4];
+    __device__ __forceinline__ void operator()(int v) const {
+        const int t = v >> 2, q = v & 3;
+        done[q] = gelu(acc[t >> 2][t & 3][q]);
+        if (q == 3)
+            *reinterpret_cast<uint2 *>(dst + 16 * (t >> 2) * kFusedPitch + 16 * (t & 3)) =
+                uint2{pack_bf16x2(done[0], done[1]), pack_bf16x2(done[2], done[3])};
+    }
+};
+struct OutputPending {
+    const floatx4 (&acc)[4][2];
+    float *dst;
+    __device__ __forceinline__ void operator()(int u) const {
+        *reinterpret_cast<floatx4 *>(dst + 16 * (u >> 1) * kFusedOut + 16 * (u & 1)) = acc[u >> 1][u & 1];
+    }
+};
+
+// Each layer runs in two phases, one per half of the rows, and in each phase the wave's MFMAs for
+// one half interleave with the VALU epilogue (GELU and bf16 stores) of the other half's previous
+// phase. Phase A of layer l multiplies rows 0-63 of layer l - 1 and writes GELU(rows 64-127 of
+// layer l - 1); phase B multiplies rows 64-127 and writes GELU(rows 0-63 of layer l). A barrier
+// follows every phase: the rows a phase writes are the ones the phase before read, and the rows
+// a phase reads are the ones the phase before wrote. Per output value the products, their K
+// order, the bias seeding and the GELU are those of the layer-by-layer kernels.
 __global__ __launch_bounds__(512, 1) void infer_fused(const FusedArgs a) {
     extern __shared__ __attribute__((aligned(16))) bf16 smem[];
     bf16 *Xs = smem;                                 // activations [128 x kFusedPitch]
@@ -1128,36 +1171,37 @@ __global__ __launch_bounds__(512, 1) void infer_fused(const FusedArgs a) {
             *reinterpret_cast<const uint4 *>(a.X + (row0 + rr) * a.in_pad + c);
     }
     __syncthreads();
+    const int nh = wave * (kFusedHid / 8), no = wave * (kFusedOut / 8);
+    // the lane's row and first column in each half of the activation tile
+    bf16 *const x_lo = Xs + r * kFusedPitch + nh + g4, *const x_hi = x_lo + 64 * kFusedPitch;
+    constexpr int kHalf = 64 * kFusedPitch, kInHalf = 64 * kFusedInPitch;
+    floatx4 lo[4][4], hi[4][4];  // a hidden layer's rows 0-63 and 64-127
     for (int h = 0; h < a.heads; ++h) {
         const bf16 *W = a.W + h * a.sW;
         const float *B = a.bias + h * a.sBias;
-        for (int l = 0; l < a.depth; ++l) {
-            floatx4 acc[8][4];
-            const int n0 = wave * (kFusedHid / 8);
-            if (l == 0)
-                fused_layer<4, kFusedK0 / kBK, kFusedInPitch>(Ins, W + a.wt[0], n0, B + a.bo[0], acc, lane);
-            else
-                fused_layer<4, kFusedHid / kBK, kFusedPitch>(Xs, W + a.wt[l], n0, B + a.bo[l], acc, lane);
-            __syncthreads();  // every wave has read the tile (and the previous head's last layer)
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const floatx4 &zz = acc[i][j];
-                    *reinterpret_cast<uint2 *>(Xs + (16 * i + r) * kFusedPitch + n0 + 16 * j + g4) =
-                        uint2{pack_bf16x2(gelu(zz[0]), gelu(zz[1])), pack_bf16x2(gelu(zz[2]), gelu(zz[3]))};
-                }
+        fused_half<4, kFusedK0 / kBK, kFusedInPitch, 0>(Ins, W + a.wt[0], nh, B + a.bo[0], lo, lane, NoPending{});
+        __syncthreads();
+        fused_half<4, kFusedK0 / kBK, kFusedInPitch, 64>(Ins + kInHalf, W + a.wt[0], nh, B + a.bo[0], hi, lane,
+                                                          GeluPending{lo, x_lo, {}});
+        __syncthreads();
+        for (int l = 1; l < a.depth; ++l) {
+            fused_half<4, kFusedHid / kBK, kFusedPitch, 64>(Xs, W + a.wt[l], nh, B + a.bo[l], lo, lane,
+                                                             GeluPending{hi, x_hi, {}});
+            __syncthreads();
+            fused_half<4, kFusedHid / kBK, kFusedPitch, 64>(Xs + kHalf, W + a.wt[l], nh, B + a.bo[l], hi, lane,
+                                                             GeluPending{lo, x_lo, {}});
             __syncthreads();
         }
-        floatx4 acc[8][2];
-        const int n0 = wave * (kFusedOut / 8);
-        fused_layer<2, kFusedHid / kBK, kFusedPitch>(Xs, W + a.wt[a.depth], n0, B + a.bo[a.depth], acc, lane);
-        float *Y = a.Yp + h * a.sYp;
+        floatx4 olo[4][2], ohi[4][2];
+        float *Y = a.Yp + h * a.sYp + (row0 + r) * kFusedOut + no + g4;  // the lane's row and first column
+        fused_half<2, kFusedHid / kBK, kFusedPitch, 64>(Xs, W + a.wt[a.depth], no, B + a.bo[a.depth], olo, lane,
+                                                         GeluPending{hi, x_hi, {}});
+        __syncthreads();
+        fused_half<2, kFusedHid / kBK, kFusedPitch, 8>(Xs + kHalf, W + a.wt[a.depth], no, B + a.bo[a.depth], ohi,
+                                                        lane, OutputPending{olo, Y});
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                *reinterpret_cast<floatx4 *>(Y + (row0 + 16 * i + r) * kFusedOut + n0 + 16 * j + g4) = acc[i][j];
+        for (int u = 0; u < 8; ++u) OutputPending{ohi, Y + 64 * kFusedOut}(u);
+        __syncthreads();
     }
 }
 
