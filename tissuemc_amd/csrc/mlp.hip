@@ -1010,14 +1010,13 @@ __global__ __launch_bounds__(256) void ensemble_reduce(const float *Yp, int head
 // only the inputs come in and each head's output layer goes out. Each wave computes 1/8 of a
 // layer's columns for all 128 rows, one half of the rows at a time, with the same MFMA as the
 // layer-by-layer kernels (swapped operands, bias-seeded accumulators, K in steps of 32 in
-// order), so its results are the same bit for bit. Its weight fragments come straight from global memory (L2) into registers, one K
-// step ahead, from a copy of the weights in fragment order (pack_fragments): a wave's fragment
-// of 16 output rows x 32 K is 1 KB of consecutive bytes, whole cache lines, where the [out x in]
-// layout would give it sixteen half lines (measured: 24% -> 32% of the MFMA peak). A lane's 16
-// bytes are the MFMA fragment itself, so the K loop has no LDS staging and no barrier. The activation fragments are read
-// from LDS (rows padded by 16 bytes: the 16 rows of a fragment fall in distinct bank groups).
-// Between layers the waves meet twice: before the GELU results overwrite the tile every wave has
-// read, and after, before the next layer reads them.
+// order), so its results are the same bit for bit. Its weight fragments come straight from
+// global memory (L2) into registers, one K step ahead, from a copy of the weights in fragment
+// order (pack_fragments): a wave's fragment of 16 output rows x 32 K is 1 KB of consecutive
+// bytes, whole cache lines, where the [out x in] layout would give it sixteen half lines. A
+// lane's 16 bytes are the MFMA fragment itself, so the K loop has no LDS staging and no barrier.
+// The activation fragments are read from LDS (rows padded by 16 bytes: the 16 rows of a fragment
+// fall in distinct bank groups).
 constexpr int kFusedRows = 128;
 constexpr int kFusedHid = 512;               // hidden width the kernel is built for
 constexpr int kFusedOut = 256;               // padded output width
