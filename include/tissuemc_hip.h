@@ -290,9 +290,10 @@ int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query)
  * layer's five encoded parameters within the box, to 1e-9), runs the ensemble, and routes the
  * stack to Monte Carlo (route = 1) if it is outside the domain, if the ensemble's spread of R_d
  * relative to max(R_d, 1e-3) exceeds gate->tau_rd, or if that of T_t relative to max(T_t, 1e-2)
- * exceeds gate->tau_tt. Stacks go
- * through in chunks, the host packing the next chunk while the device works on the current one;
- * only the answers come back. */
+ * exceeds gate->tau_tt. Stacks go through in chunks (128 x the CU count), the host packing the
+ * next chunk (on up to 8 threads of its own, started and joined inside the call, when n is above
+ * half a chunk) while that chunk's copy and the current chunk's kernels run; only the answers
+ * come back. */
 typedef struct tmc_surrogate_gate {
     tmc_stack_dist box;  /* the training distribution: encoding box and domain */
     const float *y_mean; /* [outputs] */
