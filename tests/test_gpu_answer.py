@@ -18,12 +18,24 @@ import tissuemc_amd.surrogate as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def model(gpu):
-    m = S.Surrogate(heads=4, hidden=128, depth=2, batch=512, seed=5)
+def make_model(hidden: int):
+    m = S.Surrogate(heads=4, hidden=hidden, depth=2, batch=512, seed=5)
     rng = np.random.default_rng(7)
     m.y_mean = rng.uniform(-2.0, -0.5, S.N_OUT).astype(np.float32)
     m.y_std = rng.uniform(0.2, 0.6, S.N_OUT).astype(np.float32)
+    return m
+
+
+@pytest.fixture(scope="module")
+def model(gpu):
+    m = make_model(128)  # the layer-by-layer inference kernels, chunks of 16384 stacks
+    yield m
+    m.close()
+
+
+@pytest.fixture(scope="module")
+def fused_model(gpu):
+    m = make_model(512)  # the fused inference kernel, chunks of 128 x the CU count
     yield m
     m.close()
 
@@ -40,8 +52,13 @@ def queries(n_in: int, n_ood: int):
     return out
 
 
-def test_device_answers_match_the_host_path(model):
-    qs = queries(36_000, 4_000)  # three chunks of 16384 rows, the last one partial
+@pytest.mark.parametrize("which", ["model", "fused_model"])
+def test_device_answers_match_the_host_path(request, which):
+    model = request.getfixturevalue(which)
+    # three chunks of 16384 rows for the layered model, the last one partial; several chunks
+    # and packing threads in both cases (above half a chunk)
+    n_in, n_ood = (36_000, 4_000) if which == "model" else (70_000, 6_000)
+    qs = queries(n_in, n_ood)
     dom = S.in_domain(qs)
     p = model.predict(S.features(qs))
     rel_rd, rel_tt = S.relative_spreads(p)
@@ -50,7 +67,7 @@ def test_device_answers_match_the_host_path(model):
     assert 0.2 < np.mean(route[dom]) < 0.6
     d = model.answer(qs, tau, profiles=True)
     assert np.array_equal(d["in_domain"], dom)
-    assert 0 < (~dom).sum() < 4_000 + 100
+    assert 0 < (~dom).sum() <= n_ood  # the wider box's stacks are partly inside the domain
     same = np.ones(len(qs), bool)
     for key in ("rd", "tt", "rd_spread", "tt_spread"):
         close = np.isclose(d[key], p[key], rtol=1e-12, atol=0)
@@ -74,6 +91,18 @@ def test_device_answers_edge_sizes(model):
     qs = queries(16384, 0)
     d = model.answer(qs, 0.1)
     p = model.predict(S.features(qs))
+    assert np.mean(np.isclose(d["tt"], p["tt"], rtol=1e-12)) >= 0.999
+
+
+@pytest.mark.parametrize("n", [1, 127, 32768, 32769])
+def test_fused_answers_edge_sizes(fused_model, n):
+    """One query, less than a workgroup's 128 rows, a chunk of 128 x 256 CUs exactly and one
+    more: the answers of the fused path equal the host path's (which runs the same kernel)."""
+    qs = queries(n, 0)
+    d = fused_model.answer(qs, 0.1)
+    p = fused_model.predict(S.features(qs))
+    assert d["rd"].shape == (n,)
+    assert np.mean(np.isclose(d["rd"], p["rd"], rtol=1e-12)) >= 0.999
     assert np.mean(np.isclose(d["tt"], p["tt"], rtol=1e-12)) >= 0.999
 
 
