@@ -89,6 +89,7 @@ static_assert(kLayerFields == 20, "layer record is 20 floats");
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const f32x4 lds_f32x4;
 typedef __attribute__((address_space(3))) const float lds_float;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 struct LayerRec {
     float f[kLayerFields];
 };
@@ -409,6 +410,12 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint32_t *tile = reinterpret_cast<uint32_t *>(lds + a.tile_off);
     const int off_az = kSlotAbs + a.lmax;
     const float last_z = static_cast<float>(a.nz - 1), last_r = static_cast<float>(a.nr - 1);
+    // The A(r,z) row lengths (grid and LDS tile) and the tile's LDS address, held in vector
+    // registers: the step loop's scalar registers are full, and from a spill slot each would be
+    // a v_readlane per event.
+    uint32_t nr_v = static_cast<uint32_t>(a.nr), tile_r_v = static_cast<uint32_t>(a.tile_r);
+    uint32_t tile_v = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u32 *)tile));
+    asm volatile("" : "+v"(nr_v), "+v"(tile_r_v), "+v"(tile_v));
     const float last_a = static_cast<float>(a.na - 1);
     const float last_t = static_cast<float>(a.nt - 1);
     const uint64_t seed = a.key[0];
@@ -572,19 +579,18 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                     if (kArz) {
                         const float rr = __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                         const int ir = bin_of(rr, wu.inv_dr, last_r);
-                        tile_bin = __umul24(static_cast<uint32_t>(iz), static_cast<uint32_t>(a.nr)) +
+                        tile_bin = __umul24(static_cast<uint32_t>(iz), nr_v) +
                                    static_cast<uint32_t>(ir);
                         if (iz < a.tile_z && ir < a.tile_r) {
-                            const uint32_t t = __umul24(static_cast<uint32_t>(iz),
-                                                        static_cast<uint32_t>(a.tile_r)) +
+                            const uint32_t t = __umul24(static_cast<uint32_t>(iz), tile_r_v) +
                                                static_cast<uint32_t>(ir);
                             if (kAgg) {
                                 agg_t = true;
                                 agg_ti = t;
                                 agg_bin = tile_bin;
                             } else {
-                                tile_old = __hip_atomic_fetch_add(tile + t, static_cast<uint32_t>(q),
-                                                                  __ATOMIC_RELAXED,
+                                tile_old = __hip_atomic_fetch_add((lds_u32 *)(uintptr_t)(tile_v + 4u * t),
+                                                                  static_cast<uint32_t>(q), __ATOMIC_RELAXED,
                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                                 tile_pending = true;
                             }
