@@ -438,7 +438,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     int layer = 0;
     uint32_t id_lo = 0, id_hi = 0, blk = 0, idx = 8;
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
-    uint32_t ev = 0, an = 0, bs = 0, nstep = 0;
+    uint32_t ev = 0, an = 0, bs = 0;
     uint64_t acc = 0;
     bool pending = false;  // stopped on a layer boundary, its crossing not made yet
     float opl = 0;  // kExit: optical path length so far (cm): sum of n s over the hops
@@ -479,7 +479,6 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 sleft = start ? 0.0f : sleft;
                 if (kExit) opl = start ? 0.0f : opl;
                 layer = start ? 0 : layer;
-                nstep = start ? 0u : nstep;
                 acc = start ? 0ull : acc;
                 ev = start ? 0u : ev;
                 bs = start ? 0u : bs;
@@ -638,8 +637,9 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 if (kArz && tile_pending) {
                     // A counter that wrapped carries 2^32 into the HBM bin (exact: the atomic
                     // returned the value it added to).
-                    const uint32_t lo = static_cast<uint32_t>(q);
-                    const uint64_t carry = (q >> 32) + (tile_old + lo < tile_old ? 1u : 0u);
+                    uint32_t sum;
+                    const bool wrapped = __builtin_add_overflow(tile_old, static_cast<uint32_t>(q), &sum);
+                    const uint64_t carry = (q >> 32) + (wrapped ? 1u : 0u);
                     if (carry != 0) hbm_add(a.arz + tile_bin, carry << 32);
                 }
             }
@@ -748,15 +748,27 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 } else {
                     const uint32_t rw = c == 1 ? r1 : (c == 2 ? r2 : r3);
                     ++c;
-                    if (unit_open(rw) < 0.1f)
-                        w = w / 0.1f;
-                    else
+                    if (unit_open(rw) < 0.1f) {
+                        // w / 0.1f, correctly rounded: the product by 10 and one correction
+                        // by its exact residual (4 instructions against 11 for the division)
+                        // equal the IEEE quotient for every float w in [2^-64, 1e-4), checked
+                        // exhaustively; below 2^-125 they would not (the residual underflows),
+                        // so weights under 2^-64 (albedos below ~1e-15) take the division.
+                        if (w >= 0x1p-64f) {
+                            const float q10 = w * 10.0f;
+                            w = __builtin_fmaf(__builtin_fmaf(-q10, 0.1f, w), 10.0f, q10);
+                        } else {
+                            w = w / 0.1f;
+                        }
+                    } else {
                         died = true;
+                    }
                 }
             }
             idx += c;
-            nstep += go ? 1u : 0u;
-            if (nstep >= a.step_limit && !died && !pending) {
+            // Steps made so far: with no crossing pending, every step was a drop (ev) or
+            // stopped on a boundary it has since crossed (bs).
+            if (ev + bs >= a.step_limit && !died && !pending) {
                 ++an;  // a photon this long is reported, not silently dropped
                 died = true;
             }
