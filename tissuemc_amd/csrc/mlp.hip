@@ -468,8 +468,8 @@ __global__ __launch_bounds__(256, 2) void gemm_infer_wide(const GemmArgs g) {
 // batch, the row index of both), split along K across gridDim.z, fp32 partials.
 // Tiles and waves as in gemm_nt: 128 x 128 per workgroup, 64 x 64 per wave, K in stages of 32.
 //
-// A stage holds 32 k-rows x 128 columns of each operand, copied row by row from global memory
-// into LDS (global_load_lds_dwordx4: rows of 256 bytes, 16 lanes a row, coalesced); the 16-byte
+// A stage holds 32 k-rows x 128 columns of each operand, loaded row by row from global memory
+// (16-byte loads, 16 lanes a row of 256 bytes, coalesced) and written to LDS; the 16-byte
 // chunk c of row r is stored at chunk c ^ s(r), s(r) = 2 (r & 3) + 8 ((r >> 3) & 1). The MFMA wants 8 consecutive k of one column
 // per lane, which ds_read_b64_tr_b16 delivers from this layout: within each 16 lanes, lane i
 // receives element i % 4 of the 8 bytes that lane 4 j + i / 4 addresses, as its element j
@@ -483,44 +483,27 @@ __global__ __launch_bounds__(256, 2) void gemm_infer_wide(const GemmArgs g) {
 // cycles went to bank conflicts with the swizzle 2 (r & 7), which gave both groups the same
 // banks).
 typedef short shortx4 __attribute__((ext_vector_type(4)));
-constexpr int kTnStages = 4;
-constexpr int kTnLds = kTnStages * kStageElems * 2;  // bytes
+typedef __attribute__((address_space(3))) shortx4 lds_shortx4;
+typedef __attribute__((address_space(3))) char lds_char;
+constexpr int kTnLds = 2 * kStageElems * 2;  // bytes: two stages
 __device__ __forceinline__ int tn_swz(int r) { return 2 * (r & 3) + 8 * ((r >> 3) & 1); }
 
-// Sixteen ds_read_b64_tr_b16 at LDS byte addresses a[0..15], and the wait for them, in one
-// block. Written as asm because the builtin's reads are not tied to the stage they read, so the
-// compiler waits for every global-to-LDS copy in flight (the next stages' too) before them. The
-// block ends only when the reads have landed; its results are early-clobber, so no address
-// register is reused for a result while reads are still being issued.
-__device__ __forceinline__ void tr_read16(shortx4 (&o)[16], const uint32_t (&a)[16]) {
-    asm volatile(
-        "ds_read_b64_tr_b16 %0, %16\n\t"
-        "ds_read_b64_tr_b16 %1, %17\n\t"
-        "ds_read_b64_tr_b16 %2, %18\n\t"
-        "ds_read_b64_tr_b16 %3, %19\n\t"
-        "ds_read_b64_tr_b16 %4, %20\n\t"
-        "ds_read_b64_tr_b16 %5, %21\n\t"
-        "ds_read_b64_tr_b16 %6, %22\n\t"
-        "ds_read_b64_tr_b16 %7, %23\n\t"
-        "ds_read_b64_tr_b16 %8, %24\n\t"
-        "ds_read_b64_tr_b16 %9, %25\n\t"
-        "ds_read_b64_tr_b16 %10, %26\n\t"
-        "ds_read_b64_tr_b16 %11, %27\n\t"
-        "ds_read_b64_tr_b16 %12, %28\n\t"
-        "ds_read_b64_tr_b16 %13, %29\n\t"
-        "ds_read_b64_tr_b16 %14, %30\n\t"
-        "ds_read_b64_tr_b16 %15, %31\n\t"
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5]),
-          "=&v"(o[6]), "=&v"(o[7]), "=&v"(o[8]), "=&v"(o[9]), "=&v"(o[10]), "=&v"(o[11]),
-          "=&v"(o[12]), "=&v"(o[13]), "=&v"(o[14]), "=&v"(o[15])
-        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
-          "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]), "v"(a[13]), "v"(a[14]),
-          "v"(a[15])
-        : "memory");
+// Wait for this wave's LDS accesses, then meet the other waves (no wait for global loads, which
+// stay in flight across it).
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-__global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
+// The loop. A stage's operands come from global memory into registers two stages ahead of their
+// use (16 bytes x 4 per lane), are written to one of two LDS slots one stage ahead, and the
+// fragments of stage kt + 1 are read from LDS (ds_read_b64_tr_b16, the builtin) in between stage
+// kt's MFMAs, one read after each MFMA, into a second set of registers: the LDS latency is
+// hidden behind the MFMAs, and the barrier each stage (after the stage's writes) also tells every
+// wave that the slot the next writes overwrite has been read. The reads are the builtin, which
+// the compiler waits for exactly; the staging is through registers rather than global-to-LDS
+// copies because with those in flight the compiler waits for every copy before each LDS read.
+// The MFMA sequence (operands, K order) is that of a plain loop over the stages.
+__global__ __launch_bounds__(256, 2) void gemm_tn(const GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) bf16 smem[];
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
@@ -537,7 +520,7 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
     const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 128;
     const int kspan = g.K / g.splits;
     const int kbeg = split * kspan;
-    const int nk = kspan / kBK;
+    const int nk = kspan / kBK;  // >= 1
     const bf16 *A = g.A + h * g.sA;
     const bf16 *B = g.B + h * g.sB;
 
@@ -549,7 +532,7 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
     const bf16 *a1 = A + static_cast<long long>(kbeg + lr1) * g.lda + mblk + lc1;
     const bf16 *b0 = B + static_cast<long long>(kbeg + lr0) * g.ldb + nblk + lc0;
     const bf16 *b1 = B + static_cast<long long>(kbeg + lr1) * g.ldb + nblk + lc1;
-    const int wdst = 8 * wave * 128;  // the wave's first row in a tile (elements)
+    const int wdst = 8 * wave * 128 + 8 * lane;  // the lane's 16 bytes in a stage's A tile (elements)
 
     // transposed fragment reads: lane L of group gq reads rows 8 gq + L / 4 (+ 4), columns
     // c0 + 4 (L % 4), c0 = the fragment's first column (a multiple of 16)
@@ -561,16 +544,16 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
         const int col = c0 + fcol;
         return static_cast<uint32_t>(2 * (row * 128 + (((col >> 3) ^ tn_swz(row)) << 3) + (col & 7)));
     };
-    // reads 0..3: k = 8 gq .. + 3 of fragments 0..3; reads 4..7: k = 8 gq + 4 .. + 7
-    uint32_t aoff[8], boff[8];
+    // reads 0..3: k = 8 gq .. + 3 of A's fragments 0..3; 4..7: k = 8 gq + 4 .. + 7; 8..15: B's
+    uint32_t off[16];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        aoff[i] = tr_off(fr0, wm + 16 * i);
-        aoff[4 + i] = tr_off(fr1, wm + 16 * i);
-        boff[i] = tr_off(fr0, wn + 16 * i);
-        boff[4 + i] = tr_off(fr1, wn + 16 * i);
+        off[i] = tr_off(fr0, wm + 16 * i);
+        off[4 + i] = tr_off(fr1, wm + 16 * i);
+        off[8 + i] = tr_off(fr0, wn + 16 * i) + 32 * 128 * 2;
+        off[12 + i] = tr_off(fr1, wn + 16 * i) + 32 * 128 * 2;
     }
-    const uint32_t lds0 = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_void *)smem));
+    lds_char *const lds = reinterpret_cast<lds_char *>((lds_void *)smem);
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -578,67 +561,89 @@ __global__ __launch_bounds__(256) void gemm_tn(const GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-#define TMC_TN_ISSUE(st)                                                                          \
-    do {                                                                                          \
-        bf16 *As_ = smem + ((st) % kTnStages) * kStageElems + wdst, *Bs_ = As_ + 32 * 128;        \
-        const long long ko = static_cast<long long>(st) * kBK;                                    \
-        __builtin_amdgcn_global_load_lds((global_cvoid *)(a0 + ko * g.lda), (lds_void *)As_, 16, 0, 0);        \
-        __builtin_amdgcn_global_load_lds((global_cvoid *)(a1 + ko * g.lda), (lds_void *)(As_ + 4 * 128), 16, 0, 0); \
-        __builtin_amdgcn_global_load_lds((global_cvoid *)(b0 + ko * g.ldb), (lds_void *)Bs_, 16, 0, 0);        \
-        __builtin_amdgcn_global_load_lds((global_cvoid *)(b1 + ko * g.ldb), (lds_void *)(Bs_ + 4 * 128), 16, 0, 0); \
-    } while (0)
-
-    // Stage kt's fragments are read while the MFMAs of stage kt - 1 run: the reads of the next
-    // stage are issued after this stage's MFMAs, into a second set of registers.
-    auto read_stage = [&](int st, bf16x8 (&fa)[4], bf16x8 (&fb)[4]) {
-        const uint32_t abase = lds0 + static_cast<uint32_t>((st % kTnStages) * kStageElems * 2);
-        const uint32_t bbase = abase + 32 * 128 * 2;
-        uint32_t ad[16];  // A's eight reads, then B's
+    struct Regs {
+        uint4 a0, a1, b0, b1;
+    };
+    auto gload = [&](int st) {
+        const long long ko = static_cast<long long>(st) * kBK;
+        Regs r;
+        r.a0 = *reinterpret_cast<const uint4 *>(a0 + ko * g.lda);
+        r.a1 = *reinterpret_cast<const uint4 *>(a1 + ko * g.lda);
+        r.b0 = *reinterpret_cast<const uint4 *>(b0 + ko * g.ldb);
+        r.b1 = *reinterpret_cast<const uint4 *>(b1 + ko * g.ldb);
+        return r;
+    };
+    auto swrite = [&](int slot, const Regs &r) {
+        bf16 *As_ = smem + slot * kStageElems + wdst, *Bs_ = As_ + 32 * 128;
+        *reinterpret_cast<uint4 *>(As_) = r.a0;
+        *reinterpret_cast<uint4 *>(As_ + 4 * 128) = r.a1;
+        *reinterpret_cast<uint4 *>(Bs_) = r.b0;
+        *reinterpret_cast<uint4 *>(Bs_ + 4 * 128) = r.b1;
+    };
+    auto rd = [&](int slot, int u) {
+        return __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+            reinterpret_cast<lds_shortx4 *>(lds + slot * kStageElems * 2 + off[u]));
+    };
+    // The operands are swapped (the tile is computed as B^T A, its transpose), so that in each
+    // 16 x 16 tile the lane holds row m = lane % 16 and four consecutive columns.
+    auto mfma = [&](const shortx4 (&f)[16], int i, int j) {
+        const bf16x8 fa = __builtin_bit_cast(bf16x8, __builtin_shufflevector(f[i], f[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+        const bf16x8 fb = __builtin_bit_cast(bf16x8, __builtin_shufflevector(f[8 + j], f[12 + j], 0, 1, 2, 3, 4, 5, 6, 7));
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb, fa, acc[i][j], 0, 0, 0);
+    };
+    auto mfma_all = [&](const shortx4 (&f)[16]) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            ad[u] = abase + aoff[u];
-            ad[8 + u] = bbase + boff[u];
-        }
-        shortx4 rd[16];
-        tr_read16(rd, ad);
+        for (int t = 0; t < 16; ++t) mfma(f, t >> 2, t & 3);
+    };
+    // this stage's MFMAs (fragments f) with the next stage's reads (from `slot`, into nf)
+    auto mfma_read = [&](const shortx4 (&f)[16], shortx4 (&nf)[16], int slot) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            fa[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[i], rd[4 + i], 0, 1, 2, 3, 4, 5, 6, 7));
-            fb[i] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(rd[8 + i], rd[12 + i], 0, 1, 2, 3, 4, 5, 6, 7));
+        for (int t = 0; t < 16; ++t) {
+            mfma(f, t >> 2, t & 3);
+            nf[t] = rd(slot, t);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
         }
     };
-    // The ring holds stages kt .. kt + kTnStages - 1: stage kt's slot is refilled (with stage
-    // kt + kTnStages) once every wave has read stage kt, which the barrier before stage kt + 1's
-    // reads establishes.
-    for (int st = 0; st < kTnStages && st < nk; ++st) TMC_TN_ISSUE(st);
-    stage_sync_ahead<kTnStages - 1>(nk - 1 < kTnStages - 1 ? nk - 1 : kTnStages - 1);
-    bf16x8 fa[4], fb[4];
-    read_stage(0, fa, fb);
-    for (int kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) {
-            // stages issued so far: up to min(kt + kTnStages - 1, nk - 1); wait for stage kt + 1
-            const int ahead = nk - 2 - kt < kTnStages - 2 ? nk - 2 - kt : kTnStages - 2;
-            stage_sync_ahead<kTnStages - 2>(ahead);
-            if (kt + kTnStages < nk) TMC_TN_ISSUE(kt + kTnStages);
-        }
+    // Stage s goes to LDS slot s % 2. At the top of each loop iteration (kt even): F0 holds
+    // stage kt's fragments, Y stage kt + 1's operands, X stage kt + 2's (loads in flight). Loads
+    // past the last stage re-read the last one (in bounds, unused).
+    const int last = nk - 1;
+    auto clampk = [&](int st) { return st < last ? st : last; };
+    Regs X = gload(0), Y = gload(clampk(1));
+    swrite(0, X);
+    X = gload(clampk(2));
+    lds_barrier();
+    shortx4 F0[16], F1[16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
-        if (kt + 1 < nk) {
-            bf16x8 na[4], nb[4];
-            read_stage(kt + 1, na, nb);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i] = na[i];
-                fb[i] = nb[i];
-            }
+    for (int u = 0; u < 16; ++u) F0[u] = rd(0, u);
+    int kt = 0;
+    for (; kt + 2 < last; kt += 2) {
+        swrite(1, Y);
+        Y = gload(kt + 3);
+        lds_barrier();
+        mfma_read(F0, F1, 1);
+        swrite(0, X);
+        X = gload(clampk(kt + 4));
+        lds_barrier();
+        mfma_read(F1, F0, 0);
+    }
+    // stages kt .. last remain (last - kt is 0, 1 or 2): Y holds stage kt + 1, X stage kt + 2
+    if (kt == last) {
+        mfma_all(F0);
+    } else {
+        swrite(1, Y);
+        lds_barrier();
+        mfma_read(F0, F1, 1);
+        if (kt + 1 == last) {
+            mfma_all(F1);
+        } else {
+            swrite(0, X);
+            lds_barrier();
+            mfma_read(F1, F0, 0);
+            mfma_all(F0);
         }
     }
-#undef TMC_TN_ISSUE
-    // The operands are swapped (the tile is computed as B^T A, its transpose), so in each 16 x 16
-    // tile the lane holds row m = lane % 16 and four consecutive columns: one 16-byte store.
     float *C = g.Cf + h * g.sCf + split * g.sSplit;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
