@@ -284,8 +284,8 @@ int tmc_mlp_flops(const tmc_mlp *model, uint64_t *per_step, uint64_t *per_query)
 /* Surrogate answers for layer stacks, end to end on the device (no reference counterpart).
  * The network must have 30 inputs (5 layer slots x (log10 mua, log10 mus, g, n, log10 d), each
  * mapped to [-1, 1] over gate->box, 0 in empty slots, then 5 slot masks) and >= 2 outputs, the
- * first two being R_d and T_t; every output is log10(value + floor), standardized by y_mean and
- * y_std. For each stack the call encodes the inputs on the device (in double precision, the
+ * first two being R_d and T_t; output 0 is log10(R_d + floor_rd), output 1 log10(T_t + floor_tt)
+ * and every other output log10(value + floor), each standardized by y_mean and y_std. For each stack the call encodes the inputs on the device (in double precision, the
  * host's arithmetic), checks the domain (layer count within the box's range, every present
  * layer's five encoded parameters within the box, to 1e-9), runs the ensemble, and routes the
  * stack to Monte Carlo (route = 1) if it is outside the domain, if the ensemble's spread of R_d
@@ -298,7 +298,9 @@ typedef struct tmc_surrogate_gate {
     tmc_stack_dist box;  /* the training distribution: encoding box and domain */
     const float *y_mean; /* [outputs] */
     const float *y_std;  /* [outputs] */
-    double floor;        /* outputs are log10(value + floor): 1e-7 in the pipeline */
+    double floor;        /* outputs 2.. are log10(value + floor): 1e-7 in the pipeline */
+    double floor_rd;     /* output 0 is log10(R_d + floor_rd): 3e-4 in the pipeline */
+    double floor_tt;     /* output 1 is log10(T_t + floor_tt): 3e-3 in the pipeline */
     double tau_rd;       /* spread gate of R_d */
     double tau_tt;       /* spread gate of T_t */
 } tmc_surrogate_gate;

@@ -63,7 +63,7 @@ def test_surrogate_pipeline_small(gpu, tmp_path):
 
 
 def test_pipeline_command_small(gpu, tmp_path):
-    args = argparse.Namespace(out=str(tmp_path), stacks=4096, photons=10_000, steps=400, batch=512,
+    args = argparse.Namespace(out=str(tmp_path), stacks=4096, photons=10_000, steps=400, scalar_weight=500.0, batch=512,
                               heads=4, hidden=128, depth=2, lr=2e-3, seed=3, val=512, val_photons=40_000, tau=0.0,
                               target_err=0.05, max_route=0.1, queries=2000, ood_frac=0.05, fallback_photons=20_000,
                               judge=200, judge_photons=100_000)

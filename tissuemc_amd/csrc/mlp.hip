@@ -1271,16 +1271,17 @@ struct FinishArgs {
     const uint8_t *dom;
     const float *y_mean, *y_std;  // [outputs]
     int rows, outputs;
-    double floor, tau_rd, tau_tt;
+    double floor_rd, floor_tt, tau_rd, tau_tt;
     double *res;     // [rows x 4]: rd, tt, rd_spread, tt_spread
     uint8_t *route;  // [rows]
 };
 
 // One thread per row: de-standardize R_d and T_t (float multiply, then add: no contraction to an
-// FMA, whose single rounding would differ from the host's), undo the
-// log10, and route the row to Monte Carlo if it is out of domain or a relative spread exceeds
-// its threshold (R_d's relative to max(R_d, 1e-3) tau_rd, T_t's relative to max(T_t, 1e-2)
-// tau_tt).
+// FMA, whose single rounding would differ from the host's), undo the log10 (the value is
+// max(10^y - floor, 0); its spread is the ensemble's spread of y times d value / dy =
+// ln 10 (value + floor)), and route the row to Monte Carlo if it is out of domain or a relative
+// spread exceeds its threshold (R_d's relative to max(R_d, 1e-3) tau_rd, T_t's relative to
+// max(T_t, 1e-2) tau_tt).
 __global__ __launch_bounds__(256) void finish_answers(const FinishArgs a) {
 #pragma clang fp contract(off)
     const int m = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1289,8 +1290,10 @@ __global__ __launch_bounds__(256) void finish_answers(const FinishArgs a) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const float y = a.mean[static_cast<long long>(m) * a.outputs + q] * a.y_std[q] + a.y_mean[q];
-        v[q] = fmax(exp10(static_cast<double>(y)) - a.floor, 0.0);
-        sp[q] = v[q] * 2.302585092994046 * static_cast<double>(a.spread[2 * m + q]) * static_cast<double>(a.y_std[q]);
+        const double fl = q == 0 ? a.floor_rd : a.floor_tt;
+        v[q] = fmax(exp10(static_cast<double>(y)) - fl, 0.0);
+        sp[q] = (v[q] + fl) * 2.302585092994046 * static_cast<double>(a.spread[2 * m + q]) *
+                static_cast<double>(a.y_std[q]);
     }
     const double rel_rd = sp[0] / fmax(fabs(v[0]), 1e-3);
     const double rel_tt = sp[1] / fmax(fabs(v[1]), 1e-2);
@@ -1749,7 +1752,8 @@ class MlpModel {
             hipLaunchKernelGGL(mlp::ensemble_reduce, dim3((rows + 3) / 4), dim3(256), 0, stream_, Yp_, H_, Bq_,
                                out_pad_, cfg_.outputs, 2, rows, mean_, spread_);
             mlp::FinishArgs fa{mean_, spread_, ans_dom_, ans_ymean_, ans_ystd_, rows, cfg_.outputs,
-                               gate.floor, gate.tau_rd, gate.tau_tt, ans_res_, ans_route_};
+                               gate.floor_rd, gate.floor_tt, gate.tau_rd, gate.tau_tt, ans_res_,
+                               ans_route_};
             hipLaunchKernelGGL(mlp::finish_answers, dim3((rows + 255) / 256), dim3(256), 0, stream_, fa);
             if (want_prof) {
                 const long long cells = static_cast<long long>(rows) * width;

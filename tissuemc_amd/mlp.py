@@ -22,8 +22,8 @@ class MlpConfig(C.Structure):
 
 class SurrogateGate(C.Structure):
     _fields_ = [("box", engine.TmcStackDist), ("y_mean", C.POINTER(C.c_float)),
-                ("y_std", C.POINTER(C.c_float)), ("floor", C.c_double), ("tau_rd", C.c_double),
-                ("tau_tt", C.c_double)]
+                ("y_std", C.POINTER(C.c_float)), ("floor", C.c_double), ("floor_rd", C.c_double),
+                ("floor_tt", C.c_double), ("tau_rd", C.c_double), ("tau_tt", C.c_double)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -138,12 +138,13 @@ class Mlp:
         return mean, spread, ph, sec.value
 
     def answer(self, stacks, *, y_mean: np.ndarray, y_std: np.ndarray, tau,
-               floor: float = 1e-7, box: engine.TmcStackDist | None = None,
-               profiles: bool = False) -> dict:
+               floor: float = 1e-7, floor_rd: float = 1e-7, floor_tt: float = 1e-7,
+               box: engine.TmcStackDist | None = None, profiles: bool = False) -> dict:
         """Gated surrogate answers for stacks, end to end on the device (tmc_mlp_answer): R_d,
         T_t, their ensemble spreads, the route-to-Monte-Carlo and in-domain flags, optionally the
         other outputs' values, and the device and wall-clock seconds of the call. `tau` is the
-        spread threshold of both outputs, or a pair (R_d's, T_t's)."""
+        spread threshold of both outputs, or a pair (R_d's, T_t's). Output 0 is log10(R_d +
+        floor_rd), output 1 log10(T_t + floor_tt), the others log10(value + floor)."""
         tau_rd, tau_tt = (tau, tau) if np.isscalar(tau) else tau
         arr = engine.stacks_array(stacks)
         n = len(arr)
@@ -151,7 +152,8 @@ class Mlp:
         ys = np.ascontiguousarray(y_std, np.float32)
         assert ym.shape == ys.shape == (self.outputs,)
         gate = SurrogateGate(box=box if box is not None else engine.default_stack_dist(),
-                             y_mean=_f(ym), y_std=_f(ys), floor=floor, tau_rd=tau_rd, tau_tt=tau_tt)
+                             y_mean=_f(ym), y_std=_f(ys), floor=floor, floor_rd=floor_rd,
+                             floor_tt=floor_tt, tau_rd=tau_rd, tau_tt=tau_tt)
         res = {k: np.zeros(n) for k in ("rd", "tt", "rd_spread", "tt_spread")}
         route = np.zeros(n, np.uint8)
         dom = np.zeros(n, np.uint8)

@@ -10,10 +10,13 @@ disagrees, are routed back to Monte Carlo; Monte Carlo at many more photons judg
 
 Encoding (30 inputs): per layer slot i < 5 the five parameters log10(mua), log10(mus), g, n,
 log10(d), each mapped to [-1, 1] over the training box; empty slots are 0; then a 5-entry layer
-mask. Targets (202 outputs): log10 of R_d, T_t, A_z (100 bins) and R_r (100 bins), each plus
-1e-7, where A_z and R_r are the per-photon weight fractions in each bin; outputs are
-standardized for training. R_d and T_t span orders of magnitude over the stack distribution, so
-their errors are learned (and gated) in relative terms.
+mask. Targets (202 outputs): log10(R_d + 3e-4), log10(T_t + 3e-3), and log10 of A_z (100 bins)
+and R_r (100 bins) plus 1e-7, where A_z and R_r are the per-photon weight fractions in each bin;
+outputs are standardized for training. R_d and T_t span orders of magnitude over the stack
+distribution, so their errors are learned (and gated) in relative terms; their floors are a few
+tenths of the floors their errors are judged against (R_d relative to max(R_d, 1e-3), T_t to
+max(T_t, 1e-2)), so that stacks whose T_t is far below 1e-2, whose labels at 1e5 photons are
+mostly Monte Carlo noise in log terms, do not dominate the fit.
 """
 
 from __future__ import annotations
@@ -37,7 +40,9 @@ N_IN = SLOTS * 5 + SLOTS
 NZ = NR = 100
 DZ = DR = 0.01
 N_OUT = 2 + NZ + NR
-FLOOR = 1e-7
+FLOOR = 1e-7       # profiles: log10(value + FLOOR)
+FLOOR_RD = 3e-4    # log10(R_d + FLOOR_RD)
+FLOOR_TT = 3e-3    # log10(T_t + FLOOR_TT)
 GRID = dict(dz=DZ, dr=DR, nz=NZ, nr=NR)
 
 
@@ -175,7 +180,7 @@ C_SIZE = 0  # set below (bytes of one TmcStack)
 
 
 def targets(mc: McResult) -> np.ndarray:
-    return np.concatenate([np.log10(mc.rd[:, None] + FLOOR), np.log10(mc.tt[:, None] + FLOOR),
+    return np.concatenate([np.log10(mc.rd[:, None] + FLOOR_RD), np.log10(mc.tt[:, None] + FLOOR_TT),
                            np.log10(mc.az + FLOOR), np.log10(mc.rr + FLOOR)],
                           axis=1).astype(np.float32)
 
@@ -191,7 +196,7 @@ class Surrogate:
         self.dist = engine.default_stack_dist()
 
     def fit(self, X: np.ndarray, Y: np.ndarray, steps: int, *, lr=1e-3, lr_final=1e-5,
-            scalar_weight: float = 50.0, segments: int = 20) -> dict:
+            scalar_weight: float = 500.0, segments: int = 20) -> dict:
         self.y_mean = Y.mean(axis=0).astype(np.float32)
         self.y_std = np.maximum(Y.std(axis=0), 1e-6).astype(np.float32)
         w = np.ones(N_OUT, np.float32)
@@ -216,21 +221,22 @@ class Surrogate:
     def predict(self, X: np.ndarray) -> dict:
         mean, spread, _, sec = self.net.predict(X)
         y = (mean * self.y_std + self.y_mean).astype(np.float64)
-        rd = np.maximum(10.0 ** y[:, 0] - FLOOR, 0.0)
-        tt = np.maximum(10.0 ** y[:, 1] - FLOOR, 0.0)
-        # spreads come out in log10 units; ln(10) * spread is the relative spread
+        rd = np.maximum(10.0 ** y[:, 0] - FLOOR_RD, 0.0)
+        tt = np.maximum(10.0 ** y[:, 1] - FLOOR_TT, 0.0)
+        # spreads come out in log10 units of value + floor: d value / dy = ln(10) (value + floor)
         return {"rd": rd, "tt": tt,
                 "az": np.maximum(10.0 ** y[:, 2:2 + NZ] - FLOOR, 0.0),
                 "rr": np.maximum(10.0 ** y[:, 2 + NZ:] - FLOOR, 0.0),
-                "rd_spread": rd * math.log(10) * spread[:, 0].astype(np.float64) * self.y_std[0],
-                "tt_spread": tt * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1],
+                "rd_spread": (rd + FLOOR_RD) * math.log(10) * spread[:, 0].astype(np.float64) * self.y_std[0],
+                "tt_spread": (tt + FLOOR_TT) * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1],
                 "seconds": sec}
 
     def answer(self, stacks, tau, profiles: bool = False) -> dict:
         """Gated answers end to end on the device (encoding, domain gate, ensemble, spread gate:
         tmc_mlp_answer); the same decisions as predict() + gate() on features()/in_domain()."""
         return self.net.answer(stacks, y_mean=self.y_mean, y_std=self.y_std, tau=tau,
-                               floor=FLOOR, box=self.dist, profiles=profiles)
+                               floor=FLOOR, floor_rd=FLOOR_RD, floor_tt=FLOOR_TT, box=self.dist,
+                               profiles=profiles)
 
     def save(self, path: Path) -> None:
         np.savez(path, params=self.net.get_params(), y_mean=self.y_mean, y_std=self.y_std)
@@ -364,7 +370,7 @@ def pipeline(args) -> dict:
     model = Surrogate(heads=args.heads, hidden=args.hidden, depth=args.depth, batch=args.batch,
                       seed=args.seed)
     try:
-        fit = model.fit(X[n_val:], Y[n_val:], args.steps, lr=args.lr)
+        fit = model.fit(X[n_val:], Y[n_val:], args.steps, lr=args.lr, scalar_weight=args.scalar_weight)
         per_step, per_query = model.net.flops()
         fit["tflops"] = per_step * args.steps / fit["seconds"] / 1e12
         fit["mfma_frac_bf16"] = fit["tflops"] / 2500.0
@@ -489,7 +495,9 @@ def main(argv=None) -> int:
     p.add_argument("--out", required=True)
     p.add_argument("--stacks", type=int, default=1 << 20)
     p.add_argument("--photons", type=int, default=100_000)
-    p.add_argument("--steps", type=int, default=20_000)
+    p.add_argument("--steps", type=int, default=360_000)
+    p.add_argument("--scalar-weight", type=float, default=500.0,
+                   help="loss weight of R_d and T_t (each profile bin weighs 1)")
     p.add_argument("--batch", type=int, default=4096)
     p.add_argument("--heads", type=int, default=4)
     p.add_argument("--hidden", type=int, default=512)
