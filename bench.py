@@ -187,7 +187,8 @@ def bench_small_runs(engine, reps: int = 20) -> dict:
     mus 100, g 0.9; 4e6 photons, 100 batches, 40 x 50 grid; README.md:308 quotes 3.156e9
     absorption events/s for it on a Tesla T4), each timed over `reps` calls of tmc_hip_run
     (device state allocated per call) and of tmc_hip_run_static (no allocation). Kernel time
-    from HIP events (tmc_hip_run) and wall time of the whole call."""
+    from HIP events (tmc_hip_run), the library call's own wall time (setup, copies, kernels), and
+    the wall time of the whole Python call (which also builds the reference's result schema)."""
     out = {}
     for key in ("c1", "tissue"):
         name, photons, batches, desc = WORKLOADS[key]
@@ -200,8 +201,10 @@ def bench_small_runs(engine, reps: int = 20) -> dict:
                   for i in range(reps)]
             wall = (time.perf_counter() - t0) / reps
             kern = sum(r["kernel_seconds"] for r in rs) / reps
+            call = sum(r["wall_seconds"] for r in rs) / reps
             events = sum(r["events"] for r in rs) / reps
             row[label] = {"photons_per_s_wall": photons / wall, "ms_per_run_wall": wall * 1e3,
+                          "ms_per_call_library": call * 1e3,
                           "kernel_ms": kern * 1e3, "events_per_s_kernel": events / kern,
                           "events_per_run": events}
         log(f"bench: {key} {row['alloc']['ms_per_run_wall']:.2f} ms per run "

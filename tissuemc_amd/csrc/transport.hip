@@ -928,6 +928,51 @@ template <bool P, bool R, bool A, bool T> const void *kernel_ptr() {
     return reinterpret_cast<const void *>(&transport_kernel<P, R, A, T>);
 }
 
+// A stream and its two timing events, reused across runs: creating a stream costs about 0.1 ms,
+// as much as a small run's copies. A Resident takes a set from its device's pool and returns it
+// when it is destroyed; the sets live until the process ends.
+struct StreamSet {
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+std::mutex g_stream_mutex;
+std::vector<std::vector<StreamSet>> g_stream_pool;  // per device ordinal
+
+// A set for `device`, which must be the current device.
+StreamSet take_streams(int device) {
+    {
+        std::lock_guard<std::mutex> lock(g_stream_mutex);
+        if (g_stream_pool.size() > static_cast<size_t>(device) && !g_stream_pool[device].empty()) {
+            const StreamSet s = g_stream_pool[device].back();
+            g_stream_pool[device].pop_back();
+            return s;
+        }
+    }
+    StreamSet s;
+    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (hipEventCreate(&s.e0) != hipSuccess || hipEventCreate(&s.e1) != hipSuccess) {
+        if (s.e0) (void)hipEventDestroy(s.e0);
+        (void)hipStreamDestroy(s.stream);
+        throw EngineError(TMC_ERR_RUNTIME, "hipEventCreate failed");
+    }
+    return s;
+}
+
+// Back to the pool, unless the stream was left capturing a graph (an error during capture):
+// such a set is destroyed.
+void give_streams(int device, const StreamSet &s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s.stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) {
+        (void)hipEventDestroy(s.e0);
+        (void)hipEventDestroy(s.e1);
+        (void)hipStreamDestroy(s.stream);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_stream_mutex);
+    if (g_stream_pool.size() <= static_cast<size_t>(device)) g_stream_pool.resize(static_cast<size_t>(device) + 1);
+    g_stream_pool[device].push_back(s);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ resident engine state
@@ -1050,9 +1095,16 @@ class Resident {
         const size_t U = p.units.size(), S = p.n_srows, PR = p.n_prows;
         n_units_ = U;
         n_stacks_floats_ = p.stacks.size();
-        units_ = alloc(U * sizeof(UnitRec));
-        stacks_ = alloc(p.stacks.size() * sizeof(float));
-        key_ = alloc(8 * (1 + kColdRows));  // the seed, then the flush rows' pointers
+        // the inputs in one region (one copy per upload): the unit table, the stack table, then
+        // the key buffer (the seed, then the flush rows' pointers), each at a 256-byte offset
+        auto up256 = [](size_t b) { return (std::max<size_t>(b, 8) + 255) / 256 * 256; };
+        in_stacks_off_ = up256(U * sizeof(UnitRec));
+        in_key_off_ = in_stacks_off_ + up256(p.stacks.size() * sizeof(float));
+        in_bytes_ = in_key_off_ + 8 * (1 + kColdRows);
+        char *in = static_cast<char *>(alloc(in_bytes_));
+        units_ = in;
+        stacks_ = in + in_stacks_off_;
+        key_ = in + in_key_off_;
         out_words_ = 1 /*cursor*/ + U /*next*/ + 5 * S + S * p.abs_stride +
                      (p.profiles ? PR * (p.nz + 2 * p.nr) : 0) +
                      (p.arz ? static_cast<size_t>(p.nz) * p.nr : 0) +
@@ -1108,9 +1160,10 @@ class Resident {
         n_prows_ = PR;
 
         if (!arena_) {
-            HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-            HIP_TRY(hipEventCreate(&e0_));
-            HIP_TRY(hipEventCreate(&e1_));
+            const StreamSet ss = take_streams(dev_.id);
+            stream_ = ss.stream;
+            e0_ = ss.e0;
+            e1_ = ss.e1;
         }
         const void *k = kernel();
         HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1120,9 +1173,10 @@ class Resident {
     ~Resident() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
-        if (e0_) (void)hipEventDestroy(e0_);
-        if (e1_) (void)hipEventDestroy(e1_);
-        if (stream_) (void)hipStreamDestroy(stream_);
+        if (stream_) {
+            (void)hipStreamSynchronize(stream_);
+            give_streams(dev_.id, StreamSet{stream_, e0_, e1_});
+        }
         for (void *p : allocs_) (void)hipFree(p);
     }
     Resident(const Resident &) = delete;
@@ -1132,12 +1186,14 @@ class Resident {
         if (p.units.size() != n_units_ || p.stacks.size() != n_stacks_floats_)
             throw EngineError(TMC_ERR_CONFIG, "the plan was made for another number of stacks");
         seed_ = seed;
-        HIP_TRY(hipMemcpyAsync(units_, p.units.data(), p.units.size() * sizeof(UnitRec),
-                               hipMemcpyHostToDevice, stream_));
-        HIP_TRY(hipMemcpyAsync(stacks_, p.stacks.data(), p.stacks.size() * sizeof(float),
-                               hipMemcpyHostToDevice, stream_));
         key_words_[0] = seed;
-        HIP_TRY(hipMemcpyAsync(key_, key_words_, sizeof key_words_, hipMemcpyHostToDevice, stream_));
+        // packed as the device region is laid out, then one copy (each copy from pageable host
+        // memory costs 10-30 us whatever its size)
+        host_in_.assign(in_bytes_, 0);
+        std::memcpy(host_in_.data(), p.units.data(), p.units.size() * sizeof(UnitRec));
+        std::memcpy(host_in_.data() + in_stacks_off_, p.stacks.data(), p.stacks.size() * sizeof(float));
+        std::memcpy(host_in_.data() + in_key_off_, key_words_, sizeof key_words_);
+        HIP_TRY(hipMemcpyAsync(units_, host_in_.data(), in_bytes_, hipMemcpyHostToDevice, stream_));
     }
 
     // Clear the outputs and run the transport kernel; returns the kernel time in seconds (HIP
@@ -1180,8 +1236,24 @@ class Resident {
                   uint64_t *arz, uint64_t *ra = nullptr, uint64_t *ta = nullptr,
                   uint64_t *rra = nullptr, uint64_t *tra = nullptr, uint64_t *rt = nullptr,
                   uint64_t *ttt = nullptr) {
+        // The outputs are one range of device memory from rd on. Up to kOneCopyBytes it comes
+        // back in one copy and is spread over the caller's buffers on the host (each copy from
+        // device to pageable memory costs 10-30 us whatever its size); larger outputs go
+        // straight to the caller's buffers, array by array.
+        const uint64_t *base = a_.rd;
+        const size_t span = out_words_ - static_cast<size_t>(base - static_cast<const uint64_t *>(out_));
+        const bool one_copy = span * 8 <= kOneCopyBytes;
+        if (one_copy) {
+            host_out_.resize(span);
+            HIP_TRY(hipMemcpyAsync(host_out_.data(), base, span * 8, hipMemcpyDeviceToHost, stream_));
+            HIP_TRY(hipStreamSynchronize(stream_));
+        }
         auto d2h = [&](uint64_t *dst, const uint64_t *src, size_t n) {
-            if (dst && src && n) HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream_));
+            if (!dst || !src || !n) return;
+            if (one_copy)
+                std::memcpy(dst, host_out_.data() + (src - base), n * 8);
+            else
+                HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, stream_));
         };
         const size_t S = n_srows_, PR = n_prows_;
         d2h(rd, a_.rd, S);
@@ -1264,6 +1336,10 @@ class Resident {
     int wpg_ = 16, groups_ = 0;
     size_t lds_bytes_ = 0, out_words_ = 0, n_units_ = 0, n_stacks_floats_ = 0, n_srows_ = 0, n_prows_ = 0;
     void *units_ = nullptr, *stacks_ = nullptr, *key_ = nullptr, *out_ = nullptr;
+    size_t in_bytes_ = 0, in_stacks_off_ = 0, in_key_off_ = 0;
+    std::vector<char> host_in_;       // the packed inputs of the last upload
+    std::vector<uint64_t> host_out_;  // the outputs of the last download (one-copy case)
+    static constexpr size_t kOneCopyBytes = size_t{8} << 20;
     uint64_t seed_ = 0;
     uint64_t key_words_[1 + kColdRows] = {};  // host copy of the key buffer (must outlive the copy)
     std::vector<void *> allocs_;
