@@ -184,6 +184,24 @@ def test_edge_shapes(gpu):
     assert few["photons"] == 3
 
 
+def test_large_outputs_come_back_whole(gpu):
+    """Outputs past 8 MiB are copied back array by array, smaller ones in one copy and spread on
+    the host: a 1200 x 1200 grid with A(r,z) (11.5 MB) against the same run without it (one
+    copy). A(r,z) does not change a photon's path, so every other tally agrees exactly, and the
+    large A(r,z) sums over r to A_z in every depth bin, exactly."""
+    spec = dict(load_spec("v3_three_layer"))
+    spec.update(nz=1200, nr=1200, dz=0.0005, dr=0.0005)
+    big = gpu.run(spec, photons=20_000, batches=10, seed=8, arz=True)
+    small = gpu.run(spec, photons=20_000, batches=10, seed=8)
+    fb, fs = big["fixed"], small["fixed"]
+    for key in ("rd", "tt", "absorbed", "az", "rr", "tr"):
+        assert np.array_equal(fb[key], fs[key]), key
+    assert big["events"] == small["events"]
+    arz = fb["arz"]
+    assert arz.shape == (1200, 1200)
+    assert np.array_equal(arz.sum(axis=1, dtype=np.uint64), fb["az"].sum(axis=0, dtype=np.uint64))
+
+
 def test_sixteen_layers(gpu):
     layers = [{"n": 1.33 + 0.01 * i, "mua": 0.5 + 0.1 * i, "mus": 50 + 5 * i, "g": 0.7,
                "d": 0.01} for i in range(16)]
