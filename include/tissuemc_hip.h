@@ -1,8 +1,10 @@
 /* tissuemc_hip.h — C ABI of the MI355X photon-transport engine.
  *
  * Plain C types only (no C++ or torch types), so that any FFI can bind it (ctypes, cgo, JNI,
- * N-API). The library owns all device memory and streams; the caller owns every host buffer it
- * passes in, and no pointer is retained after a call returns. Calls are synchronous and
+ * N-API). The library owns all device memory and streams (a run's stream and timing events come
+ * from a per-device pool and go back to it, so later calls skip their creation; the pool lives
+ * until the process ends); the caller owns every host buffer it passes in, and no pointer is
+ * retained after a call returns. Calls are synchronous and
  * blocking; they are safe to make one after another from one thread. Errors never cross the ABI
  * as exceptions: every entry point returns a TMC_* status and tmc_last_error() gives the message
  * (per calling thread).
