@@ -88,3 +88,25 @@ def test_gate_and_threshold_choice():
     # unreachable target: one threshold routing at most max_route
     a, b = S.choose_gate(pred, dom, e_rd + 1.0, e_tt, target=0.004, max_route=0.2)
     assert a == b and np.mean(S.gate(pred, dom, a)) <= 0.2
+
+
+def test_values_invert_targets():
+    """values() undoes targets() for every output (R_d, T_t with their own floors, the profile
+    bins with theirs), including exact zeros, to float32 rounding of the stored targets."""
+    rng = np.random.default_rng(5)
+    n = 400
+    rd = np.concatenate([rng.uniform(0.0, 0.6, n - 4), [0.0, 1e-6, 0.5, 1.0]])
+    tt = np.concatenate([10.0 ** rng.uniform(-6, 0, n - 4), [0.0, 1e-4, 0.01, 0.9]])
+    az = rng.uniform(0.0, 0.05, (n, S.NZ)) * (rng.uniform(size=(n, S.NZ)) > 0.2)
+    rr = rng.uniform(0.0, 0.05, (n, S.NR)) * (rng.uniform(size=(n, S.NR)) > 0.2)
+    mc = S.McResult(rd, tt, np.zeros(n), np.zeros(n), az, rr, np.full(n, 1e5), 0.0, 0.0)
+    v = S.values(S.targets(mc))
+    # log10 stored as float32: about 2.7e-7 relative in value + floor
+    for key, ref, floor in (("rd", rd, S.FLOOR_RD), ("tt", tt, S.FLOOR_TT), ("az", az, S.FLOOR),
+                            ("rr", rr, S.FLOOR)):
+        assert np.all(np.abs(v[key] - ref) <= 1e-6 * (ref + floor)), key
+    # each output has its own floor: R_d's and T_t's are the larger ones
+    assert S.FLOOR < S.FLOOR_RD < S.FLOOR_TT
+    y = S.targets(mc)
+    assert np.allclose(y[:, 0], np.log10(rd + S.FLOOR_RD), atol=1e-6)
+    assert np.allclose(y[:, 1], np.log10(tt + S.FLOOR_TT), atol=1e-6)

@@ -185,6 +185,16 @@ def targets(mc: McResult) -> np.ndarray:
                           axis=1).astype(np.float32)
 
 
+def values(y: np.ndarray) -> dict:
+    """The inverse of targets(): R_d, T_t, A_z and R_r from (de-standardized) outputs y [n x
+    202], each max(10^y - floor, 0)."""
+    y = np.asarray(y, np.float64)
+    return {"rd": np.maximum(10.0 ** y[:, 0] - FLOOR_RD, 0.0),
+            "tt": np.maximum(10.0 ** y[:, 1] - FLOOR_TT, 0.0),
+            "az": np.maximum(10.0 ** y[:, 2:2 + NZ] - FLOOR, 0.0),
+            "rr": np.maximum(10.0 ** y[:, 2 + NZ:] - FLOOR, 0.0)}
+
+
 # ------------------------------------------------------------------ the model
 
 class Surrogate:
@@ -220,16 +230,12 @@ class Surrogate:
 
     def predict(self, X: np.ndarray) -> dict:
         mean, spread, _, sec = self.net.predict(X)
-        y = (mean * self.y_std + self.y_mean).astype(np.float64)
-        rd = np.maximum(10.0 ** y[:, 0] - FLOOR_RD, 0.0)
-        tt = np.maximum(10.0 ** y[:, 1] - FLOOR_TT, 0.0)
+        out = values((mean * self.y_std + self.y_mean).astype(np.float64))
         # spreads come out in log10 units of value + floor: d value / dy = ln(10) (value + floor)
-        return {"rd": rd, "tt": tt,
-                "az": np.maximum(10.0 ** y[:, 2:2 + NZ] - FLOOR, 0.0),
-                "rr": np.maximum(10.0 ** y[:, 2 + NZ:] - FLOOR, 0.0),
-                "rd_spread": (rd + FLOOR_RD) * math.log(10) * spread[:, 0].astype(np.float64) * self.y_std[0],
-                "tt_spread": (tt + FLOOR_TT) * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1],
-                "seconds": sec}
+        out["rd_spread"] = (out["rd"] + FLOOR_RD) * math.log(10) * spread[:, 0].astype(np.float64) * self.y_std[0]
+        out["tt_spread"] = (out["tt"] + FLOOR_TT) * math.log(10) * spread[:, 1].astype(np.float64) * self.y_std[1]
+        out["seconds"] = sec
+        return out
 
     def answer(self, stacks, tau, profiles: bool = False) -> dict:
         """Gated answers end to end on the device (encoding, domain gate, ensemble, spread gate:
