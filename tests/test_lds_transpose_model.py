@@ -1,6 +1,6 @@
 """Host model of the weight-gradient product's LDS stage and its transposing fragment reads.
 
-`gemm_tn` (tissuemc_amd/csrc/mlp.hip) copies 32 batch rows x 128 columns of each operand into LDS
+`gemm_tn_group` (tissuemc_amd/csrc/mlp.hip) copies 32 batch rows x 128 columns of each operand into LDS
 row by row, with the 16-byte chunks of each row XOR-swizzled, and takes the MFMA fragments (8
 consecutive batch rows of one column per lane) through gfx950's `ds_read_b64_tr_b16`. This file
 models both halves on the host:

@@ -4,11 +4,11 @@
 // Network (per ensemble head): inputs -> depth x [hidden, GELU] -> outputs, GELU in its tanh
 // form. The forward and input-gradient products are `gemm_nt`: C[m, n] = sum_k A[m, k] * B[n, k]
 // with both operands K-contiguous in memory (the layout the MFMA wants per lane); the weight
-// gradient, whose K is the batch, is `gemm_tn`: C[m, n] = sum_k A[k, m] * B[k, n], reading the
+// gradient, whose K is the batch, is `gemm_tn_group`: C[m, n] = sum_k A[k, m] * B[k, n], reading the
 // activations and gradients as the other two wrote them and transposing in LDS:
 //   forward          Z = H_prev . W^T     gemm_nt  A = H_prev [batch x in],  B = Wt [out x in]
 //   input gradient   dH = dZ . W          gemm_nt  A = dZ [batch x out],     B = W  [in x out]
-//   weight gradient  dW = dZ^T . H_prev   gemm_tn  A = dZ [batch x out],     B = H_prev [batch x in]
+//   weight gradient  dW = dZ^T . H_prev   gemm_tn_group  A = dZ [batch x out], B = H_prev [batch x in]
 // so Adam keeps bf16 weights in both orientations (Wt and W) and nothing else is transposed.
 // Heads are independent networks batched along gridDim.z. Matrices are padded to multiples of
 // 128 with zeros that stay zero (zero inputs, zero loss weights). Weight gradients reduce over the
@@ -88,6 +88,8 @@ __device__ __forceinline__ float gelu_grad_s(float z, float s) {
     return __builtin_fmaf(z * s * (1.0f - s), du, s);
 }
 
+constexpr int kMaxMats = 8;  // weight matrices of a network (layers)
+
 // kInferHidden is kFwdHidden for inference: no Z and no transposed copy, so its results need no
 // LDS staging (below).
 enum Epilogue { kFwdHidden = 0, kFwdOut = 1, kBwdData = 2, kInferHidden = 4 };
@@ -97,13 +99,13 @@ struct GemmArgs {
     const bf16 *B;
     int M, N, K, lda, ldb;
     long long sA, sB;  // head strides (elements)
-    int splits;        // split-K factor (gemm_tn); blockIdx.z = head * splits + split
+    int splits;        // split-K factor (weight gradients): tile z = head * splits + split
     // epilogue operands (head strides in elements)
     const float *bias;
     long long sBias;
-    float *Cf;  // fp32 output [M x N] (Y for kFwdOut, dW partials for gemm_tn)
+    float *Cf;  // fp32 output [M x N] (Y for kFwdOut, dW partials for the weight gradients)
     long long sCf;
-    long long sSplit;  // gemm_tn: stride between split partials
+    long long sSplit;  // weight gradients: stride between split partials
     bf16 *Cb;   // bf16 output [M x N] (H or dZ)
     long long sCb;  // head stride of Cb (M * N)
     bf16 *Gd;         // kFwdHidden (training): GELU'(Z) as bf16, M x N in fragment order
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(256, 2) void gemm_infer_wide(const GemmArgs g) {
 
 // The weight gradient without transposed copies: C[m, n] = sum_k A[k, m] B[k, n] with A = dZ
 // [batch x out] and B = H [batch x in] as the forward and backward products wrote them (K = the
-// batch, the row index of both), split along K across gridDim.z, fp32 partials.
+// batch, the row index of both), split along K across workgroups, fp32 partials.
 // Tiles and waves as in gemm_nt: 128 x 128 per workgroup, 64 x 64 per wave, K in stages of 32.
 //
 // A stage holds 32 k-rows x 128 columns of each operand, loaded row by row from global memory
@@ -503,18 +505,14 @@ __device__ __forceinline__ void lds_barrier() {
 // the compiler waits for exactly; the staging is through registers rather than global-to-LDS
 // copies because with those in flight the compiler waits for every copy before each LDS read.
 // The MFMA sequence (operands, K order) is that of a plain loop over the stages.
-__global__ __launch_bounds__(256, 2) void gemm_tn(const GemmArgs g) {
+// One 128 x 128 tile of one (head, split) of the product g: logical workgroup `lid` of the
+// product's N / 128 x M / 128 x (heads x splits) tiles, column tiles fastest.
+__device__ __forceinline__ void gemm_tn_tile(const GemmArgs &g, int lid) {
     extern __shared__ __attribute__((aligned(16))) bf16 smem[];
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    // XCD-aware over the whole grid: workgroup L (launch order) runs on XCD L % 8; XCD k takes the
-    // contiguous logical workgroups [k W / 8, (k + 1) W / 8), i.e. whole (head, split) slices,
-    // whose tiles then share that slice's columns of dZ and H in one L2.
-    const int NX = static_cast<int>(gridDim.x), NY = static_cast<int>(gridDim.y);
-    const int T = NX * NY, W = T * static_cast<int>(gridDim.z);
-    int lid = static_cast<int>(blockIdx.x) + NX * (static_cast<int>(blockIdx.y) + NY * static_cast<int>(blockIdx.z));
-    if (W % 8 == 0) lid = (lid % 8) * (W / 8) + lid / 8;
+    const int NX = g.N / 128, T = NX * (g.M / 128);
     const int z = lid / T, tile = lid % T;
     const int h = z / g.splits, split = z % g.splits;
     const int mblk = (tile / NX) * 128, nblk = (tile % NX) * 128;
@@ -654,6 +652,26 @@ __global__ __launch_bounds__(256, 2) void gemm_tn(const GemmArgs g) {
     }
 }
 
+// Every weight gradient of a training step in one launch: product m takes the logical
+// workgroups [first[m], first[m + 1]). XCD-aware over the whole grid: workgroup L (launch order)
+// runs on XCD L % 8, and XCD k takes the contiguous logical workgroups [k W / 8, (k + 1) W / 8),
+// i.e. whole (head, split) slices, whose tiles then share that slice's columns of dZ and H in
+// one L2. One launch instead of one per layer lets the small products (the first layer's 64
+// tiles) run beside the others instead of alone.
+struct TnGroup {
+    GemmArgs g[kMaxMats];
+    int first[kMaxMats + 1];
+    int n;
+};
+__global__ __launch_bounds__(256, 2) void gemm_tn_group(const TnGroup grp) {
+    const int W = static_cast<int>(gridDim.x);
+    int lid = static_cast<int>(blockIdx.x);
+    if (W % 8 == 0) lid = (lid % 8) * (W / 8) + lid / 8;
+    int m = 0;
+    while (m + 1 < grp.n && lid >= grp.first[m + 1]) ++m;
+    gemm_tn_tile(grp.g[m], lid - grp.first[m]);
+}
+
 
 // Batch assembly for a block of 16 rows of head h: row m takes dataset sample idx (Philox of
 // (step, m, h) mod n); writes the standardized features as bf16 X [B x in_pad] and the targets
@@ -783,7 +801,6 @@ struct MatInfo {
     long long wt_off, w_off;  // offsets of the bf16 copies in a head's bf16 block
     int bb;                   // first column of the bias in the bias-gradient partials
 };
-constexpr int kMaxMats = 8;
 constexpr int kMaxSplits = 8;  // split-K pieces of a weight gradient
 
 struct AdamArgs {
@@ -1407,7 +1424,7 @@ class MlpModel {
         set_lds<mlp::kFwdOut>();
         set_lds<mlp::kBwdData>();
         set_lds<mlp::kInferHidden>();
-        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_tn),
+        MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_tn_group),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kTnLds));
         MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp::gemm_infer_wide<mlp::kInferHidden>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, mlp::kWideLds));
@@ -1486,9 +1503,7 @@ class MlpModel {
             G_.push_back(alloc<bf16>(H_ * BH));
             Hs_.push_back(alloc<bf16>(H_ * BH));
         }
-        for (int i = 0; i < 2; ++i) {
-            dZ_[i] = alloc<bf16>(H_ * BH);
-        }
+        for (int l = 0; l < c.depth; ++l) dZ_.push_back(alloc<bf16>(H_ * BH));
         const long long QH = static_cast<long long>(Bq_) * hid_;
         Xq_ = alloc<bf16>(H_ * static_cast<long long>(Bq_) * in_pad_);
         Hq_[0] = alloc<bf16>(H_ * QH);
@@ -1902,8 +1917,10 @@ class MlpModel {
     // dW [out x in] of matrix mi = dZ^T . H with dZ [B x out] (head stride s_dz) and H, the
     // layer's input, [B x in] (head stride s_h), split along the batch into partials that
     // adam_params sums.
-    void weight_grad(const mlp::MatInfo &mi, const bf16 *dz, long long s_dz, const bf16 *h,
-                     long long s_h) {
+    // The weight gradient of matrix mi, dW = dZ^T H (K = the batch, split in splits_ pieces),
+    // as one product of a gemm_tn_group launch.
+    mlp::GemmArgs weight_grad_args(const mlp::MatInfo &mi, const bf16 *dz, long long s_dz,
+                                   const bf16 *h, long long s_h) const {
         mlp::GemmArgs g = base_args();
         g.A = dz;
         g.lda = mi.out;
@@ -1919,9 +1936,26 @@ class MlpModel {
         g.sCf = static_cast<long long>(splits_) * P_;
         g.sSplit = P_;
         if (g.M % 128 || g.N % 128 || g.K % (mlp::kBK * g.splits))
-            throw EngineError(TMC_ERR_RUNTIME, "gemm_tn shape is not a multiple of the tile");
-        hipLaunchKernelGGL(mlp::gemm_tn, dim3(g.N / 128, g.M / 128, H_ * g.splits), dim3(256),
-                           mlp::kTnLds, stream_, g);
+            throw EngineError(TMC_ERR_RUNTIME, "weight gradient shape is not a multiple of the tile");
+        return g;
+    }
+
+    // Every matrix's weight gradient in one launch (products in the order given).
+    void weight_grads(const std::vector<mlp::GemmArgs> &gs) {
+        mlp::TnGroup grp;
+        std::memset(&grp, 0, sizeof grp);
+        if (gs.empty() || gs.size() > static_cast<size_t>(mlp::kMaxMats))
+            throw EngineError(TMC_ERR_RUNTIME, "weight gradient group of an unsupported size");
+        int total = 0;
+        for (size_t m = 0; m < gs.size(); ++m) {
+            grp.g[m] = gs[m];
+            grp.first[m] = total;
+            total += (gs[m].N / 128) * (gs[m].M / 128) * H_ * gs[m].splits;
+        }
+        grp.first[gs.size()] = total;
+        grp.n = static_cast<int>(gs.size());
+        hipLaunchKernelGGL(mlp::gemm_tn_group, dim3(static_cast<unsigned>(total)), dim3(256), mlp::kTnLds,
+                           stream_, grp);
         MLP_TRY(hipGetLastError());
     }
 
@@ -2059,10 +2093,12 @@ class MlpModel {
         MLP_TRY(hipGetLastError());
         const long long BH = static_cast<long long>(B_) * hid_;
         const int D = cfg_.depth;
-        // output layer
+        // the backward data chain (dZ of every hidden layer kept for the weight gradients), then
+        // every weight gradient in one launch
+        std::vector<mlp::GemmArgs> wg;
         {
             const auto &mo = mats_[D];
-            weight_grad(mo, dY_, static_cast<long long>(B_) * out_pad_, Hs_[D - 1], BH);
+            wg.push_back(weight_grad_args(mo, dY_, static_cast<long long>(B_) * out_pad_, Hs_[D - 1], BH));
             mlp::GemmArgs d = base_args();
             d.A = dY_;
             d.lda = out_pad_;
@@ -2075,18 +2111,17 @@ class MlpModel {
             d.K = out_pad_;
             d.Gin = G_[D - 1];
             set_bias_partials(d, mats_[D - 1]);
-            d.Cb = dZ_[0];
+            d.Cb = dZ_[D - 1];
             d.sCb = BH;
             gemm<mlp::kBwdData>(d);
         }
-        int cur = 0;
         for (int l = D - 1; l >= 0; --l) {
             const auto &mi = mats_[l];
-            weight_grad(mi, dZ_[cur], BH, l == 0 ? X_ : Hs_[l - 1],
-                        l == 0 ? static_cast<long long>(B_) * in_pad_ : BH);
+            wg.push_back(weight_grad_args(mi, dZ_[l], BH, l == 0 ? X_ : Hs_[l - 1],
+                                          l == 0 ? static_cast<long long>(B_) * in_pad_ : BH));
             if (l > 0) {
                 mlp::GemmArgs d = base_args();
-                d.A = dZ_[cur];
+                d.A = dZ_[l];
                 d.lda = hid_;
                 d.sA = BH;
                 d.B = wb_ + mi.w_off;
@@ -2097,12 +2132,12 @@ class MlpModel {
                 d.K = hid_;
                 d.Gin = G_[l - 1];
                 set_bias_partials(d, mats_[l - 1]);
-                d.Cb = dZ_[1 - cur];
+                d.Cb = dZ_[l - 1];
                 d.sCb = BH;
                 gemm<mlp::kBwdData>(d);
-                cur = 1 - cur;
             }
         }
+        weight_grads(wg);
         // Adam, or (without `update`) the summed gradients into g_
         {
             mlp::AdamArgs a;
@@ -2217,7 +2252,7 @@ class MlpModel {
     float *Yt_ = nullptr, *Yp_ = nullptr;
     std::vector<bf16 *> G_;  // GELU'(Z) of each hidden layer (bf16, fragment order)
     std::vector<bf16 *> Hs_;
-    bf16 *dZ_[2] = {nullptr, nullptr};
+    std::vector<bf16 *> dZ_;  // dZ of each hidden layer [heads x B x hidden]
     bf16 *Xq_ = nullptr, *Hq_[2] = {nullptr, nullptr};
     float *xq_ = nullptr, *mean_ = nullptr, *spread_ = nullptr;
     float *xd_ = nullptr, *yd_ = nullptr;
