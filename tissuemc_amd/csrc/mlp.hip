@@ -496,7 +496,10 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// The loop. A stage's operands come from global memory into registers two stages ahead of their
+// One 128 x 128 tile of one (head, split) of the product g: logical workgroup `lid` of the
+// product's N / 128 x M / 128 x (heads x splits) tiles, column tiles fastest.
+//
+// The K loop: a stage's operands come from global memory into registers two stages ahead of their
 // use (16 bytes x 4 per lane), are written to one of two LDS slots one stage ahead, and the
 // fragments of stage kt + 1 are read from LDS (ds_read_b64_tr_b16, the builtin) in between stage
 // kt's MFMAs, one read after each MFMA, into a second set of registers: the LDS latency is
@@ -505,8 +508,6 @@ __device__ __forceinline__ void lds_barrier() {
 // the compiler waits for exactly; the staging is through registers rather than global-to-LDS
 // copies because with those in flight the compiler waits for every copy before each LDS read.
 // The MFMA sequence (operands, K order) is that of a plain loop over the stages.
-// One 128 x 128 tile of one (head, split) of the product g: logical workgroup `lid` of the
-// product's N / 128 x M / 128 x (heads x splits) tiles, column tiles fastest.
 __device__ __forceinline__ void gemm_tn_tile(const GemmArgs &g, int lid) {
     extern __shared__ __attribute__((aligned(16))) bf16 smem[];
     const int tid = static_cast<int>(threadIdx.x);
