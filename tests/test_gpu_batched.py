@@ -63,6 +63,42 @@ def test_batched_stack_equals_single_run(gpu):
         assert np.array_equal(res["tr"][s], f["tr"].sum(axis=0, dtype=np.uint64))
 
 
+def test_batched_edge_stacks_equal_single_runs(gpu):
+    """Hand-picked stacks in one batched run: sixteen layers, one layer, one photon per batch,
+    backward scattering, a pure absorber, matched boundaries throughout, a denser medium below.
+    Each equals its single run bit for bit, and the per-stack identities hold exactly."""
+    def lay(n, mua, mus, g, d):
+        return {"n": n, "mua": mua, "mus": mus, "g": g, "d": d}
+    dicts = [
+        {"photons": 20_000, "n_above": 1.0, "n_below": 1.2,
+         "layers": [lay(1.33 + 0.01 * i, 0.5 + 0.1 * i, 50.0 + 5 * i, 0.7, 0.01) for i in range(16)]},
+        {"photons": 30_000, "n_above": 1.0, "n_below": 1.0, "layers": [lay(1.37, 1.0, 100.0, 0.9, 0.1)]},
+        {"photons": 4, "n_above": 1.0, "n_below": 1.0, "layers": [lay(1.4, 0.5, 60.0, 0.8, 0.2)]},
+        {"photons": 25_000, "n_above": 1.0, "n_below": 1.0,
+         "layers": [lay(1.4, 1.0, 100.0, -0.6, 0.05), lay(1.4, 0.5, 80.0, 0.8, 0.2)]},
+        {"photons": 25_000, "n_above": 1.0, "n_below": 1.0,
+         "layers": [lay(1.37, 0.3, 80.0, 0.85, 0.05), lay(1.5, 5.0, 0.0, 0.0, 0.02), lay(1.37, 0.3, 80.0, 0.85, 0.1)]},
+        {"photons": 25_000, "n_above": 1.4, "n_below": 1.4,
+         "layers": [lay(1.4, 1.0, 100.0, 0.9, 0.03), lay(1.4, 0.1, 20.0, 0.5, 0.03)]},
+        {"photons": 25_000, "n_above": 1.0, "n_below": 1.6, "layers": [lay(1.33, 0.5, 70.0, 0.85, 0.1)]},
+    ]
+    stacks = engine.stacks_array([engine.stack_from_dict(d) for d in dicts])
+    res = engine.run_batched(stacks, seed=17, batches_per_stack=4, **GRID)
+    for s in range(len(dicts)):
+        g = engine.run(single_spec(stacks[s]), batches=4, seed=17, stream=s)
+        L = stacks[s].num_layers
+        f = g["fixed"]
+        for key in ("rd", "tt"):
+            assert np.array_equal(res[key][s], f[key]), (s, key)
+        assert np.array_equal(res["absorbed"][s, :, :L], f["absorbed"]), s
+        assert res["events"][s].tolist() == [b["events"] for b in g["batches"]], s
+        for key in ("az", "rr", "tr"):
+            assert np.array_equal(res[key][s], f[key].sum(axis=0, dtype=np.uint64)), (s, key)
+        assert int(res["anomalies"][s].sum()) == 0
+    assert np.array_equal(res["az"].sum(axis=1, dtype=np.uint64),
+                          res["absorbed"].sum(axis=(1, 2), dtype=np.uint64))
+
+
 def test_graph_replays(gpu):
     a = small_stacks(64, seed=1)
     b = small_stacks(64, seed=2)
