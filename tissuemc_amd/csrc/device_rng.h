@@ -53,11 +53,21 @@ __host__ __device__ __forceinline__ Block4 philox4x32_10(uint32_t c0, uint32_t c
     return Block4{c0, c1, c2, c3};
 }
 
-// Uniform deviate in (0, 1) from one 32-bit word, keeping 23 bits: ((x >> 9) + 0.5) * 2^-23.
-// Every operation is exact, so the value equals the reference's float mapping
-// (include/tissuemc/philox.hpp:133-136) bit for bit.
-__device__ __forceinline__ float unit_open(uint32_t x) {
-    return __builtin_fmaf(static_cast<float>(x >> 9), 0x1p-23f, 0x1p-24f);
+// Uniform deviate in (0, 1) from one 32-bit word, keeping 23 bits: ((x >> 9) + 0.5) * 2^-23,
+// the reference's float mapping (include/tissuemc/philox.hpp:133-136), bit for bit. The top 23
+// bits become the mantissa of a float f = 1 + (x >> 9) 2^-23 in [1, 2) (one v_alignbit_b32
+// shifting the exponent bits 0x7F in above them), and f - (1 - 2^-24) is exact (both operands
+// within a factor of two of each other): two instructions instead of a shift, a conversion and
+// an fma. `expo` is kUnitExponent; a caller in a loop passes it in a vector register (the
+// instruction cannot take it as a literal), which keeps it out of the scalar registers.
+constexpr uint32_t kUnitExponent = 0x7Fu;
+constexpr float kUnitOffset = 0x1.fffffep-1f;  // 1 - 2^-24
+__device__ __forceinline__ float unit_open(uint32_t x, uint32_t expo = kUnitExponent) {
+    return __uint_as_float(__builtin_amdgcn_alignbit(expo, x, 9u)) - kUnitOffset;
 }
+
+// unit_open(x) < 0.1f exactly when x < kUnitBelowTenth: (m + 0.5) 2^-23 < 0.1f holds for the
+// 23-bit m = x >> 9 up to 838860, and 838861 * 512 = 429496832.
+constexpr uint32_t kUnitBelowTenth = 429496832u;
 
 }  // namespace tmc

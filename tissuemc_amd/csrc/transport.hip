@@ -173,6 +173,14 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// A wave-uniform 64-bit mask as this lane's predicate: the mask is the condition register
+// itself, no vector instruction.
+__device__ __forceinline__ bool lane_in(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// The active lanes where p holds. Free when p is one comparison (its result is the mask); any
+// other boolean is first written to a vector register and compared again (two instructions).
+__device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Weight -> fixed point (units of 2^-32), rounding to nearest. Weights never exceed 1; a weight
 // of exactly 1 (a ballistic photon under matched boundaries) is 2^32.
 __device__ __forceinline__ uint64_t to_fixed(float w) {
@@ -415,7 +423,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     // a v_readlane per event.
     uint32_t nr_v = static_cast<uint32_t>(a.nr), tile_r_v = static_cast<uint32_t>(a.tile_r);
     uint32_t tile_v = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u32 *)tile));
-    asm volatile("" : "+v"(nr_v), "+v"(tile_r_v), "+v"(tile_v));
+    uint32_t expo_v = kUnitExponent;  // unit_open's exponent bits
+    asm volatile("" : "+v"(nr_v), "+v"(tile_r_v), "+v"(tile_v), "+v"(expo_v));
+    // Likewise the A(r,z), R_r and T_r base addresses, used in branches of the step.
+    uint64_t arz_v = reinterpret_cast<uintptr_t>(a.arz), rr_v = reinterpret_cast<uintptr_t>(a.rr),
+             tr_v = reinterpret_cast<uintptr_t>(a.tr);
+    asm volatile("" : "+v"(arz_v), "+v"(rr_v), "+v"(tr_v));
+    uint64_t *const arz = reinterpret_cast<uint64_t *>(arz_v);
     const float last_a = static_cast<float>(a.na - 1);
     const float last_t = static_cast<float>(a.nt - 1);
     const uint64_t seed = a.key[0];
@@ -432,21 +446,22 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     if (have) load_stack(a, wu, ltab, lane);
     bool draining = false;
 
-    // ---- lane state
-    bool alive = false;
+    // ---- lane state. Which lanes hold a photon (alive_m), and which of those stopped on a layer
+    // boundary whose crossing is not made yet (pend_m), are wave-uniform masks in scalar
+    // registers rather than per-lane booleans: the wave's votes on them are then free.
+    uint64_t alive_m = 0, pend_m = 0;
     float x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 1, w = 0, sleft = 0;
     int layer = 0;
     uint32_t id_lo = 0, id_hi = 0, blk = 0, idx = 8;
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0, q7 = 0;
     uint32_t ev = 0, an = 0, bs = 0;
     uint64_t acc = 0;
-    bool pending = false;  // stopped on a layer boundary, its crossing not made yet
     float opl = 0;  // kExit: optical path length so far (cm): sum of n s over the hops
 
     while (have) {
         for (;;) {
             // ---- relaunch: hand the next ids of the wave's unit to lanes without a photon
-            uint64_t need = __ballot(!alive);
+            uint64_t need = ~alive_m;  // workgroups are whole waves: every lane is active here
             while (need != 0 && !draining) {
                 if (wu.cur >= wu.end && !claim_chunk(a, wu.u, lane, wu)) {
                     draining = true;
@@ -457,7 +472,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 const uint32_t m = avail < n ? static_cast<uint32_t>(avail) : n;
                 const uint32_t rank = lane_rank(need);
                 // launch (physics.hpp:58-66), by select on every lane: see the note on loop state.
-                const bool start = !alive && rank < m;
+                const uint64_t start_m = need & wave_mask(rank < m);
+                const bool start = lane_in(start_m);
                 const uint64_t id = wu.cur + rank;
                 const uint32_t lo = static_cast<uint32_t>(id), hi = static_cast<uint32_t>(id >> 32);
                 const Block4 blk0 = philox4x32_10(lo, hi, 0u, wu.word3, key0, key1);
@@ -483,13 +499,14 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 ev = start ? 0u : ev;
                 bs = start ? 0u : bs;
                 an = start ? 0u : an;
-                pending = start ? false : pending;
-                alive = alive || start;
+                pend_m &= ~start_m;
+                alive_m |= start_m;
                 wu.cur += m;
-                need = __ballot(!alive);
+                need = ~alive_m;
             }
 
-            if (draining && __ballot(alive) == 0) break;
+            if (draining && alive_m == 0) break;
+            const bool alive = lane_in(alive_m);
 
             // ---- one hop-drop-spin step (physics.hpp:264-277). Every lane runs it, so that no
             // divergent branch encloses the loop-carried state; a lane without a photon (only while
@@ -523,9 +540,9 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             // step_size (physics.hpp:149-158). A lane waiting at a boundary does not step: its
             // step length is 0, it draws no words and keeps its remaining step, and it takes no
             // part in the drop below.
-            const bool go = !pending;
+            const bool go = !lane_in(pend_m);
             const bool fresh = sleft == 0.0f;
-            float s = fresh ? __builtin_amdgcn_logf(unit_open(r0)) * P.f[kNl2InvMut]
+            float s = fresh ? __builtin_amdgcn_logf(unit_open(r0, expo_v)) * P.f[kNl2InvMut]
                             : sleft * P.f[kInvMut];
             uint32_t c = go && fresh ? 1u : 0u;
             s = go ? s : 0.0f;
@@ -534,7 +551,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             // hit_boundary (physics.hpp:162-178)
             const float zb = uz > 0.0f ? P.f[kZ1] : P.f[kZ0];
             const float dl = (zb - z) * __builtin_amdgcn_rcpf(uz);
-            const bool hit = go && (uz != 0.0f) && (s > dl);
+            const uint64_t hit_m = ~pend_m & wave_mask(uz != 0.0f) & wave_mask(s > dl);
+            const bool hit = lane_in(hit_m);  // go && uz != 0 && s > dl
             if (hit) {
                 sleft = (s - dl) * P.f[kMut];
                 s = dl;
@@ -594,13 +612,13 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                 tile_pending = true;
                             }
                         } else {
-                            hbm_add(a.arz + tile_bin, q);
+                            hbm_add(arz + tile_bin, q);
                         }
                     }
                 }
                 // spin (physics.hpp:116-144) with the HG sampler (physics.hpp:101-113)
-                const float xi_hg = unit_open(c ? r1 : r0);
-                const float xi_az = unit_open(c ? r2 : r1);
+                const float xi_hg = unit_open(c ? r1 : r0, expo_v);
+                const float xi_az = unit_open(c ? r2 : r1, expo_v);
                 c += 2;
                 float cost;
                 if (P.f[kHgC] == 0.0f) {
@@ -640,7 +658,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                     uint32_t sum;
                     const bool wrapped = __builtin_add_overflow(tile_old, static_cast<uint32_t>(q), &sum);
                     const uint64_t carry = (q >> 32) + (wrapped ? 1u : 0u);
-                    if (carry != 0) hbm_add(a.arz + tile_bin, carry << 32);
+                    if (carry != 0) hbm_add(arz + tile_bin, carry << 32);
                 }
             }
             if (kAgg) {
@@ -652,11 +670,10 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             // and then they cross together. The crossing branch is divergent code that the whole
             // wave issues; batching crossings issues it a few times less often. Each photon still
             // makes the same crossing with the same random word, only later.
-            pending = pending || (hit && alive);
-            const uint64_t waiting = __ballot(pending);
-            if (waiting != 0 &&
-                (__popcll(waiting) >= kCrossBatch || waiting == __ballot(alive) || draining)) {
-                if (pending) {
+            pend_m |= hit_m & alive_m;
+            if (pend_m != 0 &&
+                (__popcll(pend_m) >= kCrossBatch || pend_m == alive_m || draining)) {
+                if (lane_in(pend_m)) {
                     // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
                     ++bs;
                     const bool up = uz < 0.0f;
@@ -691,7 +708,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                             }
                         }
                     }
-                    const float xi = unit_open(c ? r1 : r0);
+                    const float xi = unit_open(c ? r1 : r0, expo_v);
                     ++c;
                     if (xi > r) {
                         ux *= eta;
@@ -708,8 +725,9 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                     const float rr =
                                         __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                                     const int ir = bin_of(rr, wu.inv_dr, last_r);
-                                    uint64_t *row = (exit_top ? a.rr : a.tr) +
-                                                    static_cast<uint64_t>(wu.prow) * a.nr;
+                                    uint64_t *row =
+                                        reinterpret_cast<uint64_t *>(exit_top ? rr_v : tr_v) +
+                                        static_cast<uint64_t>(wu.prow) * a.nr;
                                     hbm_add(row + ir, qw);
                                     if (kExit && a.na > 0) {
                                         // MCML's exit angle: from the normal, acos of the
@@ -739,16 +757,17 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                         uz = -uz;
                     }
                 }
-                pending = false;
+                pend_m = 0;
             }
+            const bool waits = lane_in(pend_m);  // still at a boundary, its crossing not made yet
             // roulette (physics.hpp:250-259, 274-276), after the step's crossing if it had one
-            if (!died && !pending && w < 1e-4f) {
+            if (!died && !waits && w < 1e-4f) {
                 if (w == 0.0f) {
                     died = true;
                 } else {
                     const uint32_t rw = c == 1 ? r1 : (c == 2 ? r2 : r3);
                     ++c;
-                    if (unit_open(rw) < 0.1f) {
+                    if (rw < kUnitBelowTenth) {  // unit_open(rw) < CHANCE = 0.1f
                         // w / 0.1f, correctly rounded: the product by 10 and one correction
                         // by its exact residual (4 instructions against 11 for the division)
                         // equal the IEEE quotient for every float w in [2^-64, 1e-4), checked
@@ -768,7 +787,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             idx += c;
             // Steps made so far: with no crossing pending, every step was a drop (ev) or
             // stopped on a boundary it has since crossed (bs).
-            if (ev + bs >= a.step_limit && !died && !pending) {
+            if (ev + bs >= a.step_limit && !died && !waits) {
                 ++an;  // a photon this long is reported, not silently dropped
                 died = true;
             }
@@ -778,7 +797,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 if (bs != 0) lds_add(slot + kSlotBoundary, bs);
                 if (an != 0) lds_add(slot + kSlotAnomalies, an);
             }
-            alive = alive && !died;
+            alive_m &= ~wave_mask(died);
         }
         // The unit is done for this wave: publish its slot, then find another unit.
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
