@@ -93,20 +93,23 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 struct LayerRec {
     float f[kLayerFields];
 };
-__device__ __forceinline__ LayerRec load_layer(uint32_t ltab, int layer) {
+// Reads 16-byte groups [kFirst, kFirst + kCount) of a layer's record into r. The step reads the
+// first three (stepping, absorption, scattering, n) and a boundary crossing the last two, so the
+// crossing's eight values are not held in registers through every step.
+template <int kFirst, int kCount>
+__device__ __forceinline__ void load_layer(LayerRec &r, uint32_t ltab, int layer) {
     const uint32_t at = ltab + __umul24(static_cast<uint32_t>(layer), kLayerFields * sizeof(float));
     const lds_f32x4 *p = (const lds_f32x4 *)(uintptr_t)at;
-    LayerRec r;
 #pragma unroll
-    for (int k = 0; k < kLayerFields / 4; ++k) {
+    for (int k = kFirst; k < kFirst + kCount; ++k) {
         const f32x4 v = p[k];
         r.f[4 * k] = v.x;
         r.f[4 * k + 1] = v.y;
         r.f[4 * k + 2] = v.z;
         r.f[4 * k + 3] = v.w;
     }
-    return r;
 }
+static_assert(kN < 12 && kNUp >= 12, "the step's groups end with n, the crossing's begin with n_above");
 
 // Stack record (floats): this header, then kLayerFields floats per layer.
 enum StackHeader { kHdrW0 = 0, kHdrInvDz, kHdrInvDr, kHdrLayers, kStackHeader };
@@ -512,7 +515,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
             // divergent branch encloses the loop-carried state; a lane without a photon (only while
             // its wave drains a unit) steps stale values and scores nothing. The layer's constants
             // are read from the wave's LDS table at every step rather than carried in registers.
-            const LayerRec P = load_layer(ltab_lds, layer);
+            LayerRec P;
+            load_layer<0, 3>(P, ltab_lds, layer);
             {
                 // Refill when fewer than 4 buffered words are left. In a wave some lane needs
                 // a block at nearly every step, so every lane computes its next block and keeps
@@ -676,6 +680,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 if (lane_in(pend_m)) {
                     // cross_or_not (physics.hpp:199-246) with Fresnel (physics.hpp:71-98)
                     ++bs;
+                    load_layer<3, 2>(P, ltab_lds, layer);
                     const bool up = uz < 0.0f;
                     const float ni = P.f[kN];
                     const float nt = up ? P.f[kNUp] : P.f[kNDown];
