@@ -316,7 +316,15 @@ __device__ __forceinline__ bool claim_chunk(const KernelArgs &a, uint32_t u, int
         off = __hip_atomic_fetch_add(a.next + u, static_cast<uint64_t>(a.chunk), __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
     off = readfirstlane_u64(off);
-    const UnitRec rec = a.units[u];
+    UnitRec rec = a.units[u];
+    // The two words the step loop uses become scalars here, so the wait for their load is made
+    // here, once per chunk. A loaded value first used inside the loop (word3 in every Philox
+    // block) makes the compiler wait at that use on every step, with vmcnt(0), which also waits
+    // for every HBM atomic the wave still has in flight. (Measured: no change in time on the
+    // three-layer, tissue and batched workloads, where other waves cover those waits; two fewer
+    // VGPRs.)
+    rec.word3 = __builtin_amdgcn_readfirstlane(rec.word3);
+    rec.prow = __builtin_amdgcn_readfirstlane(rec.prow);
     if (off >= rec.count) return false;
     wu.u = u;
     wu.cur = rec.first + off;
@@ -372,6 +380,9 @@ __device__ __forceinline__ void load_stack(const KernelArgs &a, WaveUnit &wu, fl
     wu.inv_dz = rec[kHdrInvDz];
     wu.inv_dr = rec[kHdrInvDr];
     wu.nl = static_cast<int>(rec[kHdrLayers]);
+    // The empty asm uses these four here, so the wait for their loads is made here and not in
+    // the step loop (see claim_chunk).
+    asm volatile("" : "+v"(wu.w0), "+v"(wu.inv_dz), "+v"(wu.inv_dr), "+v"(wu.nl));
     for (int j = lane; j < wu.nl * kLayerFields; j += 64) ltab[j] = rec[kStackHeader + j];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
