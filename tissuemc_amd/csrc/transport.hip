@@ -207,6 +207,13 @@ __device__ __forceinline__ void lds_add(uint64_t *p, uint64_t v) {
 __device__ __forceinline__ void hbm_add(uint64_t *p, uint64_t v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The same through a pointer marked global: an address formed in vector registers otherwise
+// makes a flat atomic, which also counts against the LDS wait counter, so the next LDS wait
+// would wait for the HBM atomic too.
+typedef __attribute__((address_space(1))) uint64_t global_u64;
+__device__ __forceinline__ void hbm_add(global_u64 *p, uint64_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ------------------------------------------------------------------ lane aggregation
 //
@@ -394,15 +401,17 @@ __device__ __forceinline__ void flush_slot(const KernelArgs &a, uint64_t *slot, 
                                            int lane) {
     uint64_t *const *cold = a.cold;
     asm volatile("" : "+s"(cold));
-    uint64_t *rows[kColdRows];
+    typedef const __attribute__((address_space(1))) uintptr_t global_addr;
+    global_addr *table = (global_addr *)(uintptr_t)cold;
+    global_u64 *rows[kColdRows];
 #pragma unroll
-    for (int k = 0; k < kColdRows; ++k) rows[k] = cold[k];
+    for (int k = 0; k < kColdRows; ++k) rows[k] = (global_u64 *)table[k];
     const int L = a.lmax;
     for (int j = lane; j < a.slot_words; j += 64) {
         const uint64_t v = slot[j];
         if (v == 0) continue;
         slot[j] = 0;
-        uint64_t *dst;
+        global_u64 *dst;
         if (j < kSlotAbs) {
             dst = rows[j] + wu.srow;  // kSlotRd .. kSlotBoundary are kColdRd .. kColdBsteps
         } else if (j < kSlotAbs + L) {
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
     uint64_t arz_v = reinterpret_cast<uintptr_t>(a.arz), rr_v = reinterpret_cast<uintptr_t>(a.rr),
              tr_v = reinterpret_cast<uintptr_t>(a.tr);
     asm volatile("" : "+v"(arz_v), "+v"(rr_v), "+v"(tr_v));
-    uint64_t *const arz = reinterpret_cast<uint64_t *>(arz_v);
+    global_u64 *const arz = (global_u64 *)(uintptr_t)arz_v;
     const float last_a = static_cast<float>(a.na - 1);
     const float last_t = static_cast<float>(a.nt - 1);
     const uint64_t seed = a.key[0];
@@ -669,11 +678,12 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                 uz = nz_ * inv;
                 if (kArz && tile_pending) {
                     // A counter that wrapped carries 2^32 into the HBM bin (exact: the atomic
-                    // returned the value it added to).
+                    // returned the value it added to). The test is the add's carry-out and the
+                    // q >= 2^32 comparison to_fixed made, both masks: one vector instruction.
                     uint32_t sum;
                     const bool wrapped = __builtin_add_overflow(tile_old, static_cast<uint32_t>(q), &sum);
-                    const uint64_t carry = (q >> 32) + (wrapped ? 1u : 0u);
-                    if (carry != 0) hbm_add(arz + tile_bin, carry << 32);
+                    if (wrapped || q >= (1ull << 32))
+                        hbm_add(arz + tile_bin, ((q >> 32) + (wrapped ? 1u : 0u)) << 32);
                 }
             }
             if (kAgg) {
@@ -741,8 +751,8 @@ __global__ __launch_bounds__(1024) void transport_kernel(const KernelArgs a) {
                                     const float rr =
                                         __builtin_amdgcn_sqrtf(__builtin_fmaf(x, x, y * y));
                                     const int ir = bin_of(rr, wu.inv_dr, last_r);
-                                    uint64_t *row =
-                                        reinterpret_cast<uint64_t *>(exit_top ? rr_v : tr_v) +
+                                    global_u64 *row =
+                                        (global_u64 *)(uintptr_t)(exit_top ? rr_v : tr_v) +
                                         static_cast<uint64_t>(wu.prow) * a.nr;
                                     hbm_add(row + ir, qw);
                                     if (kExit && a.na > 0) {
