@@ -1133,8 +1133,12 @@ class Resident {
         wpg_ = wpg;
         groups_ = t.groups > 0 ? t.groups : dev.cus;
         const uint64_t waves_total = static_cast<uint64_t>(groups_) * wpg;
+        // Ids claimed per wave at a time: about 64 chunks per wave, at most 1024 ids. The last
+        // chunks set how unevenly the waves finish (three-layer stack, 1e8 photons: 230 ms with
+        // chunks of 3051, 219 with 381-763); below a few hundred ids the claims themselves start
+        // to cost (batched stacks: 703 ms at 1024, 738 at 256).
         a_.chunk = t.chunk > 0 ? static_cast<uint32_t>(t.chunk)
-                               : static_cast<uint32_t>(std::clamp<uint64_t>(p.photons / (waves_total * 8), 64, 4096));
+                               : static_cast<uint32_t>(std::clamp<uint64_t>(p.photons / (waves_total * 64), 64, 1024));
 
         // Device memory: inputs, then everything the clear kernel zeroes in one range.
         const size_t U = p.units.size(), S = p.n_srows, PR = p.n_prows;
