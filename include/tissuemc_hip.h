@@ -81,7 +81,8 @@ typedef struct tmc_opts {
     uint32_t flags;          /* TMC_FLAG_* */
     int32_t waves_per_group; /* 0 = automatic */
     int32_t groups;          /* persistent workgroups; 0 = one per compute unit */
-    int32_t chunk;           /* photon ids claimed per wave at a time; 0 = automatic */
+    int32_t chunk;           /* photon ids claimed per wave at a time; 0 = automatic (about
+                                1/64 of a wave's share of the photons, 64 to 1024) */
     int32_t arz_tile_z;      /* depth bins of A(r,z) kept in LDS; -1 = automatic */
     int32_t arz_tile_r;      /* radial bins of A(r,z) kept in LDS; -1 = automatic */
     uint64_t step_limit;     /* steps after which a photon is ended as an anomaly; 0 = 1e9 */
